@@ -1,0 +1,379 @@
+// api.hip -- the extern "C" boundary (include/femhip.h).  Every entry point
+// validates its arguments, converts C++ failures into status codes and
+// records a message for fem_last_error(); no exception crosses the ABI.
+#include "femhip_step.h"
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+
+namespace femhip {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+  g_last_error = buf;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return FEM_ERR_OOM;
+  return FEM_ERR_HIP;
+}
+
+}  // namespace femhip
+
+using namespace femhip;
+
+#define FEM_GUARD(...)                               \
+  try {                                              \
+    __VA_ARGS__                                      \
+  } catch (const std::bad_alloc&) {                  \
+    set_error("host allocation failed");             \
+    return FEM_ERR_OOM;                              \
+  } catch (const std::exception& ex) {               \
+    set_error(ex.what());                            \
+    return FEM_ERR_INVALID;                          \
+  } catch (...) {                                    \
+    set_error("unknown C++ exception");              \
+    return FEM_ERR_INVALID;                          \
+  }
+
+extern "C" {
+
+const char* fem_last_error(void) { return g_last_error.c_str(); }
+
+int fem_abi_version(void) { return FEMHIP_ABI_VERSION; }
+
+int fem_device_count(int32_t* count) {
+  if (!count) return FEM_ERR_INVALID;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = (e == hipSuccess) ? c : 0;
+  return FEM_OK;
+}
+
+void fem_opts_default(fem_solver_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->engine = FEM_ENGINE_PERSISTENT;
+  o->precision = FEM_PREC_FP32_IR;
+  o->krylov = FEM_KRYLOV_AUTO;
+  o->nrhs = 1;
+  o->warm_start = 1;
+  o->tol = 1e-10;
+  o->inner_tol = 1e-4;
+}
+
+// ------------------------------------------------------------------ context
+
+int fem_ctx_create(const fem_mesh* mesh, int32_t bc, fem_ctx** out) {
+  FEM_GUARD({
+    if (!mesh || !out || mesh->num_vertices < 0 || mesh->num_triangles < 0 ||
+        (mesh->num_vertices > 0 && (!mesh->xyz || !mesh->on_boundary)) ||
+        (mesh->num_triangles > 0 && !mesh->triangles)) {
+      set_error("fem_ctx_create: invalid mesh");
+      return FEM_ERR_INVALID;
+    }
+    if (bc != FEM_DIRICHLET && bc != FEM_NEUMANN) {
+      set_error("fem_ctx_create: unknown boundary condition");
+      return FEM_ERR_INVALID;
+    }
+    *out = nullptr;
+    return ctx_create(mesh, bc, out);
+  })
+}
+
+int fem_ctx_destroy(fem_ctx* ctx) {
+  FEM_GUARD({
+    ctx_free(ctx);
+    return FEM_OK;
+  })
+}
+
+int fem_ctx_assemble(fem_ctx* ctx, const fem_assembly_params* params) {
+  FEM_GUARD({
+    if (!ctx) {
+      set_error("fem_ctx_assemble: null context");
+      return FEM_ERR_INVALID;
+    }
+    return ctx_assemble(ctx, params);
+  })
+}
+
+int fem_ctx_info_get(const fem_ctx* ctx, fem_ctx_info* info) {
+  if (!ctx || !info) {
+    set_error("fem_ctx_info_get: null argument");
+    return FEM_ERR_INVALID;
+  }
+  info->num_nodes = ctx->nv;
+  info->num_unknowns = ctx->n;
+  info->num_triangles = ctx->ntri;
+  info->max_row_nonzeros = ctx->max_row;
+  info->nnz = ctx->nnz;
+  return FEM_OK;
+}
+
+int fem_ctx_get_idx_map(const fem_ctx* ctx, int32_t* idx_map) {
+  if (!ctx || !idx_map) {
+    set_error("fem_ctx_get_idx_map: null argument");
+    return FEM_ERR_INVALID;
+  }
+  if (ctx->nv > 0) FEM_HIP(hipMemcpy(idx_map, ctx->d_idx, sizeof(int) * ctx->nv, hipMemcpyDeviceToHost));
+  return FEM_OK;
+}
+
+int fem_ctx_get_pattern(const fem_ctx* ctx, int32_t* row_ptr, int32_t* col_idx) {
+  if (!ctx || !row_ptr || (ctx->nnz > 0 && !col_idx)) {
+    set_error("fem_ctx_get_pattern: null argument");
+    return FEM_ERR_INVALID;
+  }
+  FEM_HIP(hipMemcpy(row_ptr, ctx->d_rp, sizeof(int) * (ctx->n + 1), hipMemcpyDeviceToHost));
+  if (ctx->nnz > 0) FEM_HIP(hipMemcpy(col_idx, ctx->d_ci, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost));
+  return FEM_OK;
+}
+
+int fem_ctx_get_matrix(const fem_ctx* ctx, int32_t which, double* values) {
+  if (!ctx || (!values && ctx->nnz > 0)) {
+    set_error("fem_ctx_get_matrix: null argument");
+    return FEM_ERR_INVALID;
+  }
+  const double* src = which == FEM_MATRIX_K ? ctx->d_K : which == FEM_MATRIX_M ? ctx->d_M : which == FEM_MATRIX_A ? ctx->d_A : nullptr;
+  if (!src) {
+    set_error("fem_ctx_get_matrix: unknown matrix");
+    return FEM_ERR_INVALID;
+  }
+  if (ctx->nnz > 0) FEM_HIP(hipMemcpy(values, src, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost));
+  return FEM_OK;
+}
+
+// ------------------------------------------------------------------ solver
+
+int fem_solver_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_solver** out) {
+  FEM_GUARD({
+    if (!ctx || !out) {
+      set_error("fem_solver_create: null argument");
+      return FEM_ERR_INVALID;
+    }
+    fem_batch* b = nullptr;
+    int rc = batch_create(&ctx, 1, opts, &b);
+    if (rc) return rc;
+    fem_solver* s = new fem_solver();
+    s->batch = b;
+    *out = s;
+    return FEM_OK;
+  })
+}
+
+int fem_solver_destroy(fem_solver* s) {
+  FEM_GUARD({
+    if (s) {
+      batch_free(s->batch);
+      delete s;
+    }
+    return FEM_OK;
+  })
+}
+
+int fem_solver_set_equation(fem_solver* s, int32_t equation, const void* params, size_t size) {
+  FEM_GUARD({
+    if (!s) {
+      set_error("null solver");
+      return FEM_ERR_INVALID;
+    }
+    return batch_set_equation(s->batch, 0, equation, params, size);
+  })
+}
+
+// Surface-space state: gather nv -> N through idx_map (CPUSolver.cpp:13-21).
+int fem_solver_set_state(fem_solver* s, int32_t field, const float* values_nv) {
+  FEM_GUARD({
+    if (!s || !values_nv) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    fem_batch* b = s->batch;
+    fem_ctx* c = b->insts[0].ctx;
+    double* d;
+    int col, nc;
+    int rc = batch_field_ptr(b, 0, field, &d, &col, &nc);
+    if (rc) return rc;
+    std::vector<int> idx(c->nv);
+    rc = fem_ctx_get_idx_map(c, idx.data());
+    if (rc) return rc;
+    std::vector<double> x((size_t)c->n * nc, 0.0);
+    for (int i = 0; i < c->nv; i++)
+      if (idx[i] >= 0)
+        for (int j = 0; j < nc; j++) x[(size_t)idx[i] * nc + j] = values_nv[(size_t)i * nc + j];
+    return batch_set_state_unknowns(b, 0, field, x.data());
+  })
+}
+
+// Scatter N -> nv, boundary entries written as 0 (CPUSolver.cpp:28-37).
+int fem_solver_get_state(fem_solver* s, int32_t field, float* values_nv) {
+  FEM_GUARD({
+    if (!s || !values_nv) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    fem_batch* b = s->batch;
+    fem_ctx* c = b->insts[0].ctx;
+    double* d;
+    int col, nc;
+    int rc = batch_field_ptr(b, 0, field, &d, &col, &nc);
+    if (rc) return rc;
+    std::vector<int> idx(c->nv);
+    rc = fem_ctx_get_idx_map(c, idx.data());
+    if (rc) return rc;
+    std::vector<double> x((size_t)c->n * nc, 0.0);
+    rc = batch_get_state_unknowns(b, 0, field, x.data());
+    if (rc) return rc;
+    for (int i = 0; i < c->nv; i++)
+      for (int j = 0; j < nc; j++)
+        values_nv[(size_t)i * nc + j] = idx[i] >= 0 ? (float)x[(size_t)idx[i] * nc + j] : 0.0f;
+    return FEM_OK;
+  })
+}
+
+int fem_solver_set_state_unknowns(fem_solver* s, int32_t field, const double* x) {
+  FEM_GUARD({
+    if (!s || !x) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    return batch_set_state_unknowns(s->batch, 0, field, x);
+  })
+}
+
+int fem_solver_get_state_unknowns(fem_solver* s, int32_t field, double* x) {
+  FEM_GUARD({
+    if (!s || !x) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    return batch_get_state_unknowns(s->batch, 0, field, x);
+  })
+}
+
+int fem_solver_advance(fem_solver* s, int32_t nsteps, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!s) {
+      set_error("null solver");
+      return FEM_ERR_INVALID;
+    }
+    return batch_advance(s->batch, nsteps, stats);
+  })
+}
+
+int fem_solver_clear(fem_solver* s) {
+  FEM_GUARD({
+    if (!s) {
+      set_error("null solver");
+      return FEM_ERR_INVALID;
+    }
+    return batch_clear(s->batch, -1);
+  })
+}
+
+int fem_solver_unstable(fem_solver* s, int32_t* unstable) {
+  FEM_GUARD({
+    if (!s || !unstable) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    int st = 0;
+    const auto& h = s->batch->insts[0];
+    if (h.d_status) FEM_HIP(hipMemcpy(&st, h.d_status, sizeof(int), hipMemcpyDeviceToHost));
+    *unstable = (st & (ST_UNSTABLE | ST_NONFINITE)) ? 1 : 0;
+    return FEM_OK;
+  })
+}
+
+int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr) {
+  FEM_GUARD({
+    if (!s || !dptr) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    double* d;
+    int col, nc;
+    int rc = batch_field_ptr(s->batch, 0, field, &d, &col, &nc);
+    if (rc) return rc;
+    *dptr = (void*)d;
+    return FEM_OK;
+  })
+}
+
+int fem_solver_sync(fem_solver* s) {
+  if (!s) return FEM_ERR_INVALID;
+  FEM_HIP(hipStreamSynchronize(s->batch->stream));
+  return FEM_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+int fem_batch_create(fem_ctx* const* ctxs, int32_t count, const fem_solver_opts* opts, fem_batch** out) {
+  FEM_GUARD({
+    if (!out) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    return batch_create(ctxs, count, opts, out);
+  })
+}
+
+int fem_batch_destroy(fem_batch* b) {
+  FEM_GUARD({
+    batch_free(b);
+    return FEM_OK;
+  })
+}
+
+int fem_batch_set_equation(fem_batch* b, int32_t instance, int32_t equation, const void* params, size_t size) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    return batch_set_equation(b, instance, equation, params, size);
+  })
+}
+
+int fem_batch_set_state_unknowns(fem_batch* b, int32_t instance, int32_t field, const double* x) {
+  FEM_GUARD({
+    if (!b || !x) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    return batch_set_state_unknowns(b, instance, field, x);
+  })
+}
+
+int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, double* x) {
+  FEM_GUARD({
+    if (!b || !x) {
+      set_error("null argument");
+      return FEM_ERR_INVALID;
+    }
+    return batch_get_state_unknowns(b, instance, field, x);
+  })
+}
+
+int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    return batch_advance(b, nsteps, stats);
+  })
+}
+
+int fem_batch_sync(fem_batch* b) {
+  if (!b) return FEM_ERR_INVALID;
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  return FEM_OK;
+}
+
+}  // extern "C"

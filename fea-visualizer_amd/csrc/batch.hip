@@ -1,0 +1,676 @@
+// batch.hip -- host side of the solver: per-instance buffers, system-matrix
+// forming, state transfer, the team plan (which workgroups advance which
+// instance) and the launch of the persistent stepping kernel.
+//
+// A single solver is a batch of one instance.  The system matrices of the
+// reference (CPUSolver.cpp:77,95,113,133,136) are formed once per parameter
+// change instead of once per step: they only depend on the parameters, so the
+// result is the same and the per-step cost is the solve alone.
+#include "femhip_step.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace femhip {
+
+namespace {
+
+struct Coef {
+  double cM[kMaxK], cK[kMaxK], cA[kMaxK];
+};
+
+template <typename TI>
+__global__ void k_form(int n, const int* rp, const int* diag, const double* M, const double* Kk, const double* A,
+                       int kv, Coef cf, double* S64, TI* Si, double* dinv64, TI* dinvi) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int s = rp[r]; s < rp[r + 1]; s++) {
+    const double m = M[s], kk = Kk[s], a = A[s];
+    for (int j = 0; j < kv; j++) {
+      double val = 0.0;
+      if (cf.cM[j] != 0.0) val += cf.cM[j] * m;
+      if (cf.cK[j] != 0.0) val += cf.cK[j] * kk;
+      if (cf.cA[j] != 0.0) val += cf.cA[j] * a;
+      S64[(size_t)s * kv + j] = val;
+      Si[(size_t)s * kv + j] = (TI)val;
+    }
+  }
+  // Eigen DiagonalPreconditioner: 1/a_jj, or 1 when absent or zero
+  const int d = diag[r];
+  for (int j = 0; j < kv; j++) {
+    const double dv = (d >= 0) ? S64[(size_t)d * kv + j] : 0.0;
+    const double inv = (dv != 0.0) ? 1.0 / dv : 1.0;
+    dinv64[(size_t)r * kv + j] = inv;
+    dinvi[(size_t)r * kv + j] = (TI)inv;
+  }
+}
+
+__global__ void k_reset(TeamBar* bars, int nteams, int* abort_flag, Inst* insts, int ninst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nteams) bars[i].count = 0u;
+  if (i == 0) *abort_flag = 0;
+  if (i < ninst) {
+    insts[i].counters[0] = 0;
+    insts[i].counters[1] = 0;
+    insts[i].counters[2] = 0;
+    *insts[i].status = 0;
+  }
+}
+
+// Column copy between an [n][nrhs] host-layout array and an [n][k] device array.
+__global__ void k_put_cols(double* dst, int k, int col0, const double* src, int ncols, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ncols) return;
+  long long r = i / ncols;
+  int j = (int)(i % ncols);
+  dst[r * k + col0 + j] = src[i];
+}
+__global__ void k_get_cols(const double* src, int k, int col0, double* dst, int ncols, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ncols) return;
+  long long r = i / ncols;
+  int j = (int)(i % ncols);
+  dst[i] = src[r * k + col0 + j];
+}
+
+void free_inst(InstHost& h) {
+  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_u, h.d_v, h.d_a, h.d_x, h.d_b, h.d_vec,
+                  h.d_status, h.d_counters, h.d_last_res};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  fem_ctx* c = h.ctx;
+  h = InstHost();
+  h.ctx = c;
+}
+
+size_t inner_bytes(const fem_batch* b) { return b->opts.precision == FEM_PREC_FP32_IR ? 4 : 8; }
+
+int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
+  free_inst(h);
+  int rc = 0;
+  const long long n = h.ctx->n, nnz = h.ctx->nnz;
+  const size_t tb = inner_bytes(b);
+  h.k = k;
+  h.kv = kv;
+  h.d_S64 = dalloc<double>((size_t)nnz * kv, &rc);
+  h.d_Si = (void*)dalloc<char>((size_t)nnz * kv * tb, &rc);
+  h.d_dinv64 = dalloc<double>((size_t)n * kv, &rc);
+  h.d_dinvi = (void*)dalloc<char>((size_t)n * kv * tb, &rc);
+  h.d_u = dalloc<double>((size_t)n * k, &rc);
+  h.d_v = dalloc<double>((size_t)n * k, &rc);
+  h.d_a = dalloc<double>((size_t)n * k, &rc);
+  h.d_x = dalloc<double>((size_t)n * k, &rc);
+  h.d_b = dalloc<double>((size_t)n * k, &rc);
+  h.d_vec = (void*)dalloc<char>((size_t)n * k * tb * 11, &rc);
+  h.d_status = dalloc<int>(4, &rc);
+  h.d_counters = dalloc<long long>(4, &rc);
+  h.d_last_res = dalloc<double>(kMaxK, &rc);
+  if (rc) return rc;
+  const size_t vb = (size_t)n * k * sizeof(double);
+  FEM_HIP(hipMemsetAsync(h.d_u, 0, vb, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_v, 0, vb, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_a, 0, vb, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_x, 0, vb, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_b, 0, vb, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_vec, 0, (size_t)n * k * tb * 11, b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_status, 0, 4 * sizeof(int), b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_counters, 0, 4 * sizeof(long long), b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_last_res, 0, kMaxK * sizeof(double), b->stream));
+  h.dirty = true;
+  return 0;
+}
+
+int form_inst(fem_batch* b, InstHost& h) {
+  fem_ctx* c = h.ctx;
+  Coef cf;
+  std::memset(&cf, 0, sizeof(cf));
+  const double dt = h.dt;
+  switch (h.eq) {
+    case FEM_HEAT:  // M/dt + c K                                  CPUSolver.cpp:77
+      cf.cM[0] = 1.0 / dt;
+      cf.cK[0] = h.p1;
+      break;
+    case FEM_ADVECTION_DIFFUSION:  // M/dt + c K - A                CPUSolver.cpp:95
+      cf.cM[0] = 1.0 / dt;
+      cf.cK[0] = h.p1;
+      cf.cA[0] = -1.0;
+      break;
+    case FEM_WAVE:  // M/dt + c^2 K dt                              CPUSolver.cpp:113
+      cf.cM[0] = 1.0 / dt;
+      cf.cK[0] = h.p1 * h.p1 * dt;
+      break;
+    case FEM_REACTION_DIFFUSION:  // M/dt + Du K ; M/dt + Dv K      CPUSolver.cpp:133,136
+      cf.cM[0] = 1.0 / dt;
+      cf.cK[0] = h.p1;
+      cf.cM[1] = 1.0 / dt;
+      cf.cK[1] = h.p2;
+      break;
+    case FEM_WAVE_NEWMARK:  // M + beta dt^2 c^2 K
+      cf.cM[0] = 1.0;
+      cf.cK[0] = h.p2 * dt * dt * h.p1 * h.p1;
+      break;
+    default:
+      set_error("form: equation not set");
+      return FEM_ERR_STATE;
+  }
+  if (c->n > 0) {
+    if (inner_bytes(b) == 4)
+      hipLaunchKernelGGL(k_form<float>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n, c->d_rp,
+                         c->d_diag, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (float*)h.d_Si, h.d_dinv64,
+                         (float*)h.d_dinvi);
+    else
+      hipLaunchKernelGGL(k_form<double>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n, c->d_rp,
+                         c->d_diag, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (double*)h.d_Si, h.d_dinv64,
+                         (double*)h.d_dinvi);
+    FEM_HIP(hipGetLastError());
+  }
+  h.formed_version = c->version;
+  h.dirty = false;
+  return 0;
+}
+
+void fill_dev(fem_batch* b, InstHost& h) {
+  fem_ctx* c = h.ctx;
+  Inst& I = h.dev;
+  std::memset(&I, 0, sizeof(I));
+  const fem_solver_opts& o = b->opts;
+  I.n = c->n;
+  I.nnz = (int)c->nnz;
+  I.k = h.k;
+  I.kv = h.kv;
+  I.eq = h.eq;
+  int kr = o.krylov;
+  if (kr == FEM_KRYLOV_AUTO) kr = (h.eq == FEM_ADVECTION_DIFFUSION) ? FEM_KRYLOV_BICGSTAB : FEM_KRYLOV_CG;
+  I.krylov = (kr == FEM_KRYLOV_BICGSTAB) ? 1 : 0;
+  I.warm = o.warm_start;
+  I.max_inner = o.max_inner > 0 ? o.max_inner : 200;
+  I.max_sweeps = o.max_sweeps > 0 ? o.max_sweeps : 30;
+  I.max_iters = o.max_iters > 0 ? o.max_iters : std::max(1, 2 * c->n);  // Eigen default 2N
+  I.weighting = h.weighting;
+  const double tol = o.tol > 0 ? o.tol : 1e-10;
+  const double itol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  I.tol2 = tol * tol;
+  I.inner_tol2 = itol * itol;
+  I.dt = h.dt;
+  I.p1 = h.p1;
+  I.p2 = h.p2;
+  I.p3 = h.p3;
+  I.p4 = h.p4;
+  I.rp = c->d_rp;
+  I.ci = c->d_ci;
+  I.M64 = c->d_M;
+  I.K64 = c->d_K;
+  I.S64 = h.d_S64;
+  I.Si = h.d_Si;
+  I.dinv64 = h.d_dinv64;
+  I.dinvi = h.d_dinvi;
+  I.lumped = c->d_lumped;
+  I.u = h.d_u;
+  I.v = h.d_v;
+  I.a = h.d_a;
+  I.x = h.d_x;
+  I.b = h.d_b;
+  char* base = (char*)h.d_vec;
+  const size_t vb = (size_t)c->n * h.k * inner_bytes(b);
+  void** slots[] = {&I.xi, &I.r, &I.z, &I.p0, &I.p1b, &I.q, &I.rh, &I.v0, &I.v1, &I.s, &I.t};
+  for (int q = 0; q < 11; q++) *slots[q] = (void*)(base + q * vb);
+  I.status = h.d_status;
+  I.counters = h.d_counters;
+  I.last_res = h.d_last_res;
+}
+
+void free_plan(fem_batch* b) {
+  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_bars, b->d_partial, b->d_scal, b->d_abort};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  b->d_insts = nullptr;
+  b->d_teams = nullptr;
+  b->d_wg_team = nullptr;
+  b->d_team_inst = nullptr;
+  b->d_bars = nullptr;
+  b->d_partial = nullptr;
+  b->d_scal = nullptr;
+  b->d_abort = nullptr;
+  b->plan_ready = false;
+}
+
+// Team plan.  Work of an instance ~ (nnz + n) * k per Krylov iteration.  With
+// one instance the team takes ~4 rows per lane, capped by the co-resident
+// grid.  With many, instances heavier than the per-workgroup share get teams
+// proportional to their work; the rest are packed longest-first onto
+// single-workgroup teams (each walks its instances in turn).
+int build_plan(fem_batch* b) {
+  free_plan(b);
+  const int ninst = (int)b->insts.size();
+  const int k = b->insts[0].k;
+  for (auto& h : b->insts)
+    if (h.k != k) {
+      set_error("batch instances must share the number of right-hand side columns");
+      return FEM_ERR_INVALID;
+    }
+  const int inner_fp32 = inner_bytes(b) == 4;
+  int maxb = 0;
+  int rc = step_max_coresident(k, inner_fp32, &maxb);
+  if (rc) return rc;
+  if (maxb <= 0) {
+    set_error("persistent kernel cannot be resident on this device");
+    return FEM_ERR_HIP;
+  }
+  std::vector<double> work(ninst);
+  double W = 0;
+  for (int i = 0; i < ninst; i++) {
+    work[i] = (double)(b->insts[i].ctx->nnz + b->insts[i].ctx->n) * k + 1.0;
+    W += work[i];
+  }
+  std::vector<std::vector<int>> teams;  // instance lists
+  std::vector<int> sizes;
+  if (b->opts.team_size > 0 && (long long)b->opts.team_size * ninst <= maxb) {
+    for (int i = 0; i < ninst; i++) {
+      teams.push_back({i});
+      sizes.push_back(b->opts.team_size);
+    }
+  } else if (ninst == 1) {
+    const long long lanes = (long long)b->insts[0].ctx->n * k;
+    int g = (int)std::max(1LL, (lanes + 4 * kBlock - 1) / (4 * kBlock));
+    g = std::min(g, maxb);
+    teams.push_back({0});
+    sizes.push_back(g);
+  } else {
+    const double share = W / maxb;
+    std::vector<int> small;
+    int used = 0;
+    for (int i = 0; i < ninst; i++) {
+      int g = (int)std::floor(work[i] / share);
+      if (g >= 2) {
+        teams.push_back({i});
+        sizes.push_back(g);
+        used += g;
+      } else {
+        small.push_back(i);
+      }
+    }
+    if (used > maxb) {  // rounding guard
+      double f = (double)maxb / (used + (double)small.size());
+      used = 0;
+      for (auto& g : sizes) {
+        g = std::max(1, (int)(g * f));
+        used += g;
+      }
+    }
+    int bins = std::max(1, std::min((int)small.size(), maxb - used));
+    std::sort(small.begin(), small.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::vector<double> load(bins, 0.0);
+    std::vector<std::vector<int>> lists(bins);
+    for (int i : small) {
+      int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lists[m].push_back(i);
+      load[m] += work[i];
+    }
+    for (int m = 0; m < bins; m++)
+      if (!lists[m].empty()) {
+        teams.push_back(lists[m]);
+        sizes.push_back(1);
+      }
+  }
+  const int nteams = (int)teams.size();
+  int grid = 0;
+  for (int g : sizes) grid += g;
+  if (grid > maxb) {
+    set_error("team plan exceeds the co-resident grid");
+    return FEM_ERR_INVALID;
+  }
+  std::vector<Team> ht(nteams);
+  std::vector<int> wg_team(grid), team_inst;
+  int wg = 0;
+  for (int t = 0; t < nteams; t++) {
+    ht[t].first_wg = wg;
+    ht[t].size = sizes[t];
+    ht[t].inst_begin = (int)team_inst.size();
+    for (int i : teams[t]) team_inst.push_back(i);
+    ht[t].inst_end = (int)team_inst.size();
+    for (int q = 0; q < sizes[t]; q++) wg_team[wg++] = t;
+  }
+  b->d_insts = dalloc<Inst>(ninst, &rc);
+  b->d_teams = dalloc<Team>(nteams, &rc);
+  b->d_wg_team = dalloc<int>(grid, &rc);
+  b->d_team_inst = dalloc<int>(team_inst.size(), &rc);
+  b->d_bars = dalloc<TeamBar>(nteams, &rc);
+  b->d_partial = dalloc<double>((size_t)grid * kMaxRed, &rc);
+  b->d_scal = dalloc<double>((size_t)nteams * kMaxRed, &rc);
+  b->d_abort = dalloc<int>(1, &rc);
+  if (rc) return rc;
+  int poff = 0;
+  for (int t = 0; t < nteams; t++) {
+    ht[t].bar = b->d_bars + t;
+    ht[t].partial = b->d_partial + (size_t)poff * kMaxRed;
+    ht[t].scal = b->d_scal + (size_t)t * kMaxRed;
+    ht[t].abort_flag = b->d_abort;
+    poff += sizes[t];
+  }
+  std::vector<Inst> hi(ninst);
+  for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
+  FEM_HIP(hipMemsetAsync(b->d_bars, 0, sizeof(TeamBar) * nteams, b->stream));
+  FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipMemcpyAsync(b->d_teams, ht.data(), sizeof(Team) * nteams, hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipMemcpyAsync(b->d_wg_team, wg_team.data(), sizeof(int) * grid, hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipMemcpyAsync(b->d_team_inst, team_inst.data(), sizeof(int) * team_inst.size(), hipMemcpyHostToDevice,
+                         b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  b->launch.teams = b->d_teams;
+  b->launch.wg_team = b->d_wg_team;
+  b->launch.insts = b->d_insts;
+  b->launch.team_inst = b->d_team_inst;
+  b->launch.grid = grid;
+  b->launch.k = k;
+  b->launch.inner_fp32 = inner_fp32;
+  b->nteams = nteams;
+  b->team_sizes = sizes;
+  b->plan_ready = true;
+  return 0;
+}
+
+}  // namespace
+
+void batch_free(fem_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& h : b->insts) free_inst(h);
+  free_plan(b);
+  if (b->d_plain) (void)hipFree(b->d_plain);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, fem_batch** out) {
+  if (count <= 0 || !ctxs) {
+    set_error("batch_create: no contexts");
+    return FEM_ERR_INVALID;
+  }
+  fem_batch* b = new fem_batch();
+  fem_opts_default(&b->opts);
+  if (opts) b->opts = *opts;
+  if (b->opts.nrhs <= 0) b->opts.nrhs = 1;
+  const int nr = b->opts.nrhs;
+  if (nr > kMaxK || (nr & (nr - 1)) != 0) {
+    delete b;
+    set_error("nrhs must be 1, 2, 4, 8 or 16");
+    return FEM_ERR_INVALID;
+  }
+  if (b->opts.precision != FEM_PREC_FP64 && b->opts.precision != FEM_PREC_FP32_IR) {
+    delete b;
+    set_error("unknown precision");
+    return FEM_ERR_INVALID;
+  }
+  b->nrhs = nr;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e != hipSuccess) {
+    batch_free(b);
+    return hip_fail(e, "batch_create");
+  }
+  b->insts.resize(count);
+  for (int i = 0; i < count; i++) {
+    if (!ctxs[i]) {
+      batch_free(b);
+      set_error("batch_create: null context");
+      return FEM_ERR_INVALID;
+    }
+    b->insts[i].ctx = ctxs[i];
+  }
+  *out = b;
+  return 0;
+}
+
+int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_t size) {
+  if (inst < 0 || inst >= (int)b->insts.size() || !params) {
+    set_error("set_equation: bad instance or params");
+    return FEM_ERR_INVALID;
+  }
+  InstHost& h = b->insts[inst];
+  double dt = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0;
+  int w = 0;
+  switch (eq) {
+    case FEM_HEAT: {
+      if (size != sizeof(fem_heat_params)) goto bad_size;
+      auto* p = (const fem_heat_params*)params;
+      dt = p->time_step;
+      p1 = p->conductivity;
+    } break;
+    case FEM_ADVECTION_DIFFUSION: {
+      if (size != sizeof(fem_advdiff_params)) goto bad_size;
+      auto* p = (const fem_advdiff_params*)params;
+      dt = p->time_step;
+      p1 = p->c;
+    } break;
+    case FEM_WAVE: {
+      if (size != sizeof(fem_wave_params)) goto bad_size;
+      auto* p = (const fem_wave_params*)params;
+      dt = p->time_step;
+      p1 = p->c;
+    } break;
+    case FEM_REACTION_DIFFUSION: {
+      if (size != sizeof(fem_rd_params)) goto bad_size;
+      auto* p = (const fem_rd_params*)params;
+      dt = p->time_step;
+      p1 = p->Du;
+      p2 = p->Dv;
+      p3 = p->feed_rate;
+      p4 = p->kill_rate;
+      w = p->reaction_weighting;
+      if (b->nrhs != 1) {
+        set_error("Gray-Scott needs nrhs == 1 (u and v are the two columns)");
+        return FEM_ERR_INVALID;
+      }
+    } break;
+    case FEM_WAVE_NEWMARK: {
+      if (size != sizeof(fem_newmark_params)) goto bad_size;
+      auto* p = (const fem_newmark_params*)params;
+      dt = p->time_step;
+      p1 = p->c;
+      p2 = p->beta;
+      p3 = p->gamma;
+    } break;
+    default:
+      set_error("unknown equation");
+      return FEM_ERR_INVALID;
+  }
+  if (!(dt > 0.0) || !std::isfinite(dt)) {
+    set_error("time_step must be positive");
+    return FEM_ERR_INVALID;
+  }
+  {
+    const int k = (eq == FEM_REACTION_DIFFUSION) ? 2 : b->nrhs;
+    const int kv = (eq == FEM_REACTION_DIFFUSION) ? 2 : 1;
+    if (h.k != k || h.kv != kv || !h.d_u) {
+      int rc = alloc_inst(b, h, k, kv);
+      if (rc) return rc;
+      b->plan_ready = false;
+    }
+    if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w)
+      h.dirty = true;
+    if (h.eq != eq) b->plan_ready = false;
+    h.eq = eq;
+    h.dt = dt;
+    h.p1 = p1;
+    h.p2 = p2;
+    h.p3 = p3;
+    h.p4 = p4;
+    h.weighting = w;
+    return 0;
+  }
+bad_size:
+  set_error("parameter struct size does not match the equation");
+  return FEM_ERR_INVALID;
+}
+
+// Map a state field to (device array, first column, column count).
+int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* ncols) {
+  if (inst < 0 || inst >= (int)b->insts.size()) {
+    set_error("bad instance");
+    return FEM_ERR_INVALID;
+  }
+  InstHost& h = b->insts[inst];
+  if (!h.d_u) {
+    set_error("set_equation must be called before the state is accessed");
+    return FEM_ERR_STATE;
+  }
+  if (h.eq == FEM_REACTION_DIFFUSION) {
+    if (field != FEM_FIELD_U && field != FEM_FIELD_V) goto bad;
+    *ptr = h.d_u;
+    *col = field == FEM_FIELD_U ? 0 : 1;
+    *ncols = 1;
+    return 0;
+  }
+  *col = 0;
+  *ncols = h.k;
+  switch (field) {
+    case FEM_FIELD_U: *ptr = h.d_u; return 0;
+    case FEM_FIELD_V: *ptr = h.d_v; return 0;
+    case FEM_FIELD_A: *ptr = h.d_a; return 0;
+    default: break;
+  }
+bad:
+  set_error("bad field");
+  return FEM_ERR_INVALID;
+}
+
+int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x) {
+  double* d;
+  int col, nc;
+  int rc = batch_field_ptr(b, inst, field, &d, &col, &nc);
+  if (rc) return rc;
+  InstHost& h = b->insts[inst];
+  const long long n = h.ctx->n;
+  if (n == 0) return 0;
+  double* tmp = dalloc<double>((size_t)n * nc, &rc);
+  if (!tmp) return rc;
+  FEM_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * n * nc, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  (void)hipFree(tmp);
+  return 0;
+}
+
+int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x) {
+  double* d;
+  int col, nc;
+  int rc = batch_field_ptr(b, inst, field, &d, &col, &nc);
+  if (rc) return rc;
+  InstHost& h = b->insts[inst];
+  const long long n = h.ctx->n;
+  if (n == 0) return 0;
+  double* tmp = dalloc<double>((size_t)n * nc, &rc);
+  if (!tmp) return rc;
+  hipLaunchKernelGGL(k_get_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
+  FEM_HIP(hipMemcpyAsync(x, tmp, sizeof(double) * n * nc, hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  (void)hipFree(tmp);
+  return 0;
+}
+
+int batch_clear(fem_batch* b, int inst) {
+  for (int i = 0; i < (int)b->insts.size(); i++) {
+    if (inst >= 0 && i != inst) continue;
+    InstHost& h = b->insts[i];
+    if (!h.d_u) continue;
+    const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
+    FEM_HIP(hipMemsetAsync(h.d_u, 0, vb, b->stream));
+    FEM_HIP(hipMemsetAsync(h.d_v, 0, vb, b->stream));
+    FEM_HIP(hipMemsetAsync(h.d_a, 0, vb, b->stream));
+    FEM_HIP(hipMemsetAsync(h.d_x, 0, vb, b->stream));
+    FEM_HIP(hipMemsetAsync(h.d_status, 0, sizeof(int), b->stream));
+  }
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// Prepare matrices and the plan; shared by both engines.
+int batch_prepare(fem_batch* b, bool need_plan) {
+  for (auto& h : b->insts) {
+    if (h.eq < 0 || !h.d_u) {
+      set_error("advance: set_equation was not called for every instance");
+      return FEM_ERR_STATE;
+    }
+    if (h.dirty || h.formed_version != h.ctx->version) {
+      int rc = form_inst(b, h);
+      if (rc) return rc;
+      b->plan_ready = false;
+    }
+    fill_dev(b, h);
+  }
+  if (need_plan && !b->plan_ready) return build_plan(b);
+  return 0;
+}
+
+int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
+  long long its = 0, sw = 0, steps = 0;
+  int status = 0;
+  double res = 0.0;
+  for (auto& h : b->insts) {
+    long long cnt[4];
+    int st[4];
+    double lr[kMaxK];
+    FEM_HIP(hipMemcpyAsync(cnt, h.d_counters, sizeof(cnt), hipMemcpyDeviceToHost, b->stream));
+    FEM_HIP(hipMemcpyAsync(st, h.d_status, sizeof(st), hipMemcpyDeviceToHost, b->stream));
+    FEM_HIP(hipMemcpyAsync(lr, h.d_last_res, sizeof(lr), hipMemcpyDeviceToHost, b->stream));
+    FEM_HIP(hipStreamSynchronize(b->stream));
+    its += cnt[0];
+    sw += cnt[1];
+    steps = std::max(steps, cnt[2]);
+    status |= st[0];
+    for (int j = 0; j < h.k; j++) res = std::max(res, lr[j]);
+  }
+  if (stats) {
+    stats->iterations = its;
+    stats->sweeps = sw;
+    stats->steps = steps;
+    stats->residual = res;
+    stats->ms = ms;
+    stats->status = status;
+  }
+  if (status & ST_BARRIER_TIMEOUT) {
+    set_error("device barrier wait timed out");
+    return FEM_ERR_TIMEOUT;
+  }
+  if (status & (ST_UNSTABLE | ST_NONFINITE)) return FEM_DIVERGED;
+  if (status & ST_NOT_CONVERGED) return FEM_NOT_CONVERGED;
+  return FEM_OK;
+}
+
+int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
+  if (nsteps < 0) {
+    set_error("nsteps must be >= 0");
+    return FEM_ERR_INVALID;
+  }
+  if (b->opts.engine == FEM_ENGINE_PLAIN) return plain_advance(b, nsteps, stats);
+  int rc = batch_prepare(b, true);
+  if (rc) return rc;
+  const int ninst = (int)b->insts.size();
+  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(b->nteams, ninst), kBlock)), dim3(kBlock), 0, b->stream,
+                     b->d_bars, b->nteams, b->d_abort, b->d_insts, ninst);
+  FEM_HIP(hipGetLastError());
+  FEM_HIP(hipEventRecord(b->ev0, b->stream));
+  if (nsteps > 0) {
+    rc = launch_step(b->launch, nsteps, b->stream);
+    if (rc) return rc;
+  }
+  FEM_HIP(hipEventRecord(b->ev1, b->stream));
+  FEM_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  int abort_flag = 0;
+  FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
+  rc = collect_stats(b, stats, ms);
+  if (abort_flag) {
+    set_error("device barrier wait timed out; results of this call are invalid");
+    return FEM_ERR_TIMEOUT;
+  }
+  return rc;
+}
+
+}  // namespace femhip

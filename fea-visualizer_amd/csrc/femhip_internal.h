@@ -1,0 +1,129 @@
+// femhip_internal.h -- shared host/device definitions of the MI355X (gfx950)
+// FEM backend.  Not part of the public C ABI (include/femhip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "femhip.h"
+
+namespace femhip {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kWave = 64;
+constexpr int kMaxK = 16;            // right-hand sides per instance
+constexpr int kMaxSlots = 4;         // reduction slots per phase
+constexpr int kMaxRed = kMaxSlots * kMaxK;
+
+// Equation ids (first four as the reference enum, FEMContext.hpp:9-14).
+enum Eq : int {
+  EQ_HEAT = 0,
+  EQ_WAVE = 1,
+  EQ_ADVDIFF = 2,
+  EQ_GS = 3,
+  EQ_NEWMARK = 4,
+  EQ_SOLVE = 5,  // plain solve S x = b (b supplied), used by the adjoint
+};
+
+// Status bits accumulated per instance.
+enum : int {
+  ST_NOT_CONVERGED = 1,
+  ST_UNSTABLE = 2,      // reference rule: u > 1e4 || v > 1e4
+  ST_NONFINITE = 4,     // NaN / Inf in the state (not caught by the reference)
+  ST_BARRIER_TIMEOUT = 8,
+};
+
+// Team barrier state: one per team, 128-byte separated counters.
+struct alignas(128) TeamBar {
+  unsigned int count;
+  unsigned int pad0[31];
+  unsigned int gen;
+  unsigned int pad1[31];
+};
+
+// Device-side description of one instance (a mesh, its parameters and its
+// state) advanced by one team of workgroups inside the persistent kernel.
+struct Inst {
+  int n, nnz, k, kv;         // rows, nonzeros, rhs columns, value columns (1 or k)
+  int eq, krylov;            // krylov: 0 CG, 1 BiCGSTAB
+  int warm;                  // initial guess = previous primary variable
+  int max_inner, max_sweeps, max_iters;
+  int weighting;             // Gray-Scott reaction: 0 nodal (reference), 1 lumped mass
+  int pad_;
+  double tol2, inner_tol2;
+  double dt, p1, p2, p3, p4; // equation parameters
+  const int* rp;
+  const int* ci;
+  const double* M64;
+  const double* K64;
+  const double* S64;     // [nnz][kv]
+  const void* Si;        // [nnz][kv] inner precision (float or double)
+  const double* dinv64;  // [n][kv]
+  const void* dinvi;     // [n][kv] inner precision
+  const double* lumped;  // [n]
+  double* u;             // state [n][k]  (Gray-Scott: u,v interleaved, k = 2)
+  double* v;
+  double* a;
+  double* x;             // FP64 solution [n][k]
+  double* b;             // FP64 right-hand side [n][k]
+  void* xi;              // inner-precision correction [n][k]
+  void* r;
+  void* z;
+  void* p0;              // direction, double-buffered
+  void* p1b;
+  void* q;
+  void* rh;              // BiCGSTAB shadow residual
+  void* v0;              // BiCGSTAB v, double-buffered
+  void* v1;
+  void* s;
+  void* t;
+  int* status;           // outcome bits
+  long long* counters;   // [0] iterations, [1] sweeps, [2] steps
+  double* last_res;      // [k] final relative residual of the last solve
+};
+
+// A team: a contiguous range of workgroups that advances a list of instances
+// one after the other, synchronising through its own barrier.
+struct Team {
+  int first_wg, size;
+  int inst_begin, inst_end;  // range in the team-instance list
+  TeamBar* bar;
+  double* partial;           // [size][kMaxRed]
+  double* scal;              // [kMaxRed] reduced values of the last barrier
+  int* abort_flag;
+};
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define FEM_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+  } while (0)
+
+template <typename T>
+inline T* dalloc(size_t count, int* rc) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *rc = hip_fail(e, "hipMalloc");
+    return nullptr;
+  }
+  return static_cast<T*>(p);
+}
+
+inline unsigned int div_up(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }
+
+// Device exclusive scan (int32 -> int32 offsets, total returned), scan.hip.
+int exclusive_scan_i32(const int* in, int* out, int n, int* total, hipStream_t st);
+
+}  // namespace femhip

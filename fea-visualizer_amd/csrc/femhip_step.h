@@ -1,0 +1,88 @@
+// femhip_step.h -- host-side batch / solver objects and the launch plan of the
+// persistent stepping kernel.
+#pragma once
+
+#include "femhip_ctx.h"
+
+namespace femhip {
+
+struct StepLaunch {
+  const Team* teams = nullptr;
+  const int* wg_team = nullptr;
+  const Inst* insts = nullptr;
+  const int* team_inst = nullptr;
+  int grid = 0;
+  int k = 1;
+  int inner_fp32 = 1;
+};
+
+int launch_step(const StepLaunch& L, int nsteps, hipStream_t st);
+int step_max_coresident(int k, int inner_fp32, int* blocks);
+
+// Per-instance host bookkeeping.
+struct InstHost {
+  fem_ctx* ctx = nullptr;
+  int eq = -1;
+  double dt = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;
+  int weighting = 0;
+  int k = 0;                 // columns in the kernel (Gray-Scott: 2)
+  int kv = 1;                // value columns
+  long long formed_version = -1;
+  bool dirty = true;         // matrices need (re)forming
+  double* d_S64 = nullptr;
+  void* d_Si = nullptr;
+  double* d_dinv64 = nullptr;
+  void* d_dinvi = nullptr;
+  double* d_u = nullptr;
+  double* d_v = nullptr;
+  double* d_a = nullptr;
+  double* d_x = nullptr;
+  double* d_b = nullptr;
+  void* d_vec = nullptr;     // slab of inner-precision vectors
+  int* d_status = nullptr;
+  long long* d_counters = nullptr;
+  double* d_last_res = nullptr;
+  Inst dev{};                // mirror of the device descriptor
+};
+
+}  // namespace femhip
+
+struct fem_batch {
+  std::vector<femhip::InstHost> insts;
+  fem_solver_opts opts{};
+  int nrhs = 1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // launch plan (device arrays)
+  bool plan_ready = false;
+  femhip::StepLaunch launch;
+  femhip::Inst* d_insts = nullptr;
+  femhip::Team* d_teams = nullptr;
+  int* d_wg_team = nullptr;
+  int* d_team_inst = nullptr;
+  femhip::TeamBar* d_bars = nullptr;
+  double* d_partial = nullptr;
+  double* d_scal = nullptr;
+  int* d_abort = nullptr;
+  int nteams = 0;
+  std::vector<int> team_sizes;  // for reporting
+  // plain-engine scratch
+  double* d_plain = nullptr;
+  long long plain_n = 0;
+};
+
+struct fem_solver {
+  fem_batch* batch = nullptr;
+};
+
+namespace femhip {
+int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, fem_batch** out);
+void batch_free(fem_batch* b);
+int batch_set_equation(fem_batch* b, int inst, int equation, const void* params, size_t size);
+int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x);
+int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x);
+int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
+int batch_clear(fem_batch* b, int inst);
+int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
+int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
+}  // namespace femhip

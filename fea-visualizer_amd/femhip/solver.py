@@ -1,0 +1,341 @@
+"""Host-side mirror of the reference's FEM interface over the C ABI.
+
+``FEMContext`` mirrors include/FEM/FEMContext.hpp:24-59 (mesh, boundary
+condition, idx_map, assembled matrices, per-equation parameters) and
+``HIPSolver`` mirrors the abstract ``Solver`` (include/FEM/Solver.hpp:8-15):
+``advance_time()``, ``clear_values()`` and ``has_numerical_instability()``.
+Parameter classes mirror include/FEM/EquationParameters.hpp:4-49, with the
+reference's defaults.  All arithmetic runs in the HIP library; this module
+only moves arrays across the boundary.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = [
+    "HeatParameters", "AdvectionDiffusionParameters", "WaveParameters",
+    "ReactionDiffusionParameters", "NewmarkParameters", "SolverOptions",
+    "FEMContext", "HIPSolver", "Batch", "Equation", "BoundaryCondition",
+]
+
+
+class Equation:
+    """enum class Equation (FEMContext.hpp:9-14) plus Newmark-beta."""
+    Heat = L.FEM_HEAT
+    Wave = L.FEM_WAVE
+    Advection_Diffusion = L.FEM_ADVECTION_DIFFUSION
+    Reaction_Diffusion = L.FEM_REACTION_DIFFUSION
+    Wave_Newmark = L.FEM_WAVE_NEWMARK
+
+
+class BoundaryCondition:
+    """enum class BoundaryCondition (FEMContext.hpp:16-19)."""
+    Dirichlet = L.FEM_DIRICHLET
+    Neumann = L.FEM_NEUMANN
+
+
+@dataclass
+class HeatParameters:  # EquationParameters.hpp:8-15
+    time_step: float = 0.01
+    conductivity: float = 0.05
+
+    def to_c(self):
+        return L.FEM_HEAT, L.fem_heat_params(self.time_step, self.conductivity)
+
+
+@dataclass
+class AdvectionDiffusionParameters:  # EquationParameters.hpp:17-26
+    time_step: float = 0.001
+    c: float = 0.25
+    velocity: Sequence[float] = (1.0, 0.0, 0.0)
+
+    def to_c(self):
+        return L.FEM_ADVECTION_DIFFUSION, L.fem_advdiff_params(self.time_step, self.c)
+
+
+@dataclass
+class WaveParameters:  # EquationParameters.hpp:28-35
+    time_step: float = 0.05
+    c: float = 0.05
+
+    def to_c(self):
+        return L.FEM_WAVE, L.fem_wave_params(self.time_step, self.c)
+
+
+@dataclass
+class ReactionDiffusionParameters:  # EquationParameters.hpp:37-49
+    time_step: float = 0.001
+    Du: float = 0.08
+    Dv: float = 0.04
+    feed_rate: float = 0.035
+    kill_rate: float = 0.06
+    reaction_weighting: int = L.FEM_REACTION_NODAL  # reference: nodal (CPUSolver.cpp:134,137)
+
+    def to_c(self):
+        return L.FEM_REACTION_DIFFUSION, L.fem_rd_params(
+            self.time_step, self.Du, self.Dv, self.feed_rate, self.kill_rate, self.reaction_weighting)
+
+
+@dataclass
+class NewmarkParameters:  # no reference counterpart
+    time_step: float = 2e-4
+    c: float = 1.0
+    beta: float = 0.25
+    gamma: float = 0.5
+
+    def to_c(self):
+        return L.FEM_WAVE_NEWMARK, L.fem_newmark_params(self.time_step, self.c, self.beta, self.gamma)
+
+
+@dataclass
+class SolverOptions:
+    engine: int = L.FEM_ENGINE_PERSISTENT
+    precision: int = L.FEM_PREC_FP32_IR
+    krylov: int = L.FEM_KRYLOV_AUTO
+    nrhs: int = 1
+    warm_start: int = 1
+    max_iters: int = 0
+    max_inner: int = 0
+    max_sweeps: int = 0
+    team_size: int = 0
+    tol: float = 1e-10
+    inner_tol: float = 1e-4
+
+    def to_c(self):
+        o = L.fem_solver_opts()
+        for name, _ in L.fem_solver_opts._fields_:
+            setattr(o, name, getattr(self, name))
+        return o
+
+
+class FEMContext:
+    """Mesh + boundary condition + assembled K, M, A on the GPU."""
+
+    def __init__(self, mesh: dict, boundary_condition: int = BoundaryCondition.Dirichlet,
+                 velocity=(1.0, 0.0, 0.0), velocity_e=None, normalize_velocity=True, kappa_e=None):
+        self._lib = L.load()
+        self.mesh = mesh
+        self.boundary_condition = boundary_condition
+        self.equation = Equation.Wave  # FEMContext.hpp:31 default
+        self.parameters = {
+            Equation.Heat: HeatParameters(),
+            Equation.Advection_Diffusion: AdvectionDiffusionParameters(),
+            Equation.Wave: WaveParameters(),
+            Equation.Reaction_Diffusion: ReactionDiffusionParameters(),
+            Equation.Wave_Newmark: NewmarkParameters(),
+        }
+        self._keep = {}
+        xyz = np.ascontiguousarray(mesh["xyz"], dtype=np.float32)
+        tri = np.ascontiguousarray(mesh["tri"], dtype=np.uint32)
+        onb = np.ascontiguousarray(mesh["on_boundary"], dtype=np.uint8)
+        m = L.fem_mesh(xyz.shape[0], tri.shape[0], L.ptr(xyz, C.c_float), L.ptr(tri, C.c_uint32),
+                       L.ptr(onb, C.c_uint8))
+        h = C.c_void_p()
+        L.check(self._lib.fem_ctx_create(C.byref(m), int(boundary_condition), C.byref(h)))
+        self.handle = h
+        self.assemble_matrices(velocity, velocity_e, normalize_velocity, kappa_e)
+
+    def assemble_matrices(self, velocity=None, velocity_e=None, normalize_velocity=True, kappa_e=None):
+        """FEMContext::assemble_matrices (FEMContext.cpp:57-64) on the GPU."""
+        if velocity is None:
+            velocity = self.parameters[Equation.Advection_Diffusion].velocity
+        p = L.fem_assembly_params()
+        for q in range(3):
+            p.velocity[q] = float(velocity[q])
+        p.normalize_velocity = 1 if normalize_velocity else 0
+        if velocity_e is not None:
+            ve = np.ascontiguousarray(velocity_e, dtype=np.float64).reshape(-1)
+            self._keep["ve"] = ve
+            p.velocity_e = L.ptr(ve, C.c_double)
+        if kappa_e is not None:
+            ke = np.ascontiguousarray(kappa_e, dtype=np.float64).reshape(-1)
+            self._keep["ke"] = ke
+            p.kappa_e = L.ptr(ke, C.c_double)
+        L.check(self._lib.fem_ctx_assemble(self.handle, C.byref(p)))
+
+    def info(self):
+        i = L.fem_ctx_info()
+        L.check(self._lib.fem_ctx_info_get(self.handle, C.byref(i)))
+        return i
+
+    def num_nodes(self):
+        return self.info().num_nodes
+
+    def num_unknowns(self):
+        return self.info().num_unknowns
+
+    def num_max_nonzeros_per_row(self):
+        return self.info().max_row_nonzeros
+
+    @property
+    def idx_map(self):
+        out = np.empty(self.num_nodes(), dtype=np.int32)
+        L.check(self._lib.fem_ctx_get_idx_map(self.handle, L.ptr(out, C.c_int32)))
+        return out
+
+    def pattern(self):
+        i = self.info()
+        rp = np.empty(i.num_unknowns + 1, dtype=np.int32)
+        ci = np.empty(max(1, i.nnz), dtype=np.int32)
+        L.check(self._lib.fem_ctx_get_pattern(self.handle, L.ptr(rp, C.c_int32), L.ptr(ci, C.c_int32)))
+        return rp, ci[: i.nnz]
+
+    def matrix(self, which: int):
+        i = self.info()
+        out = np.empty(max(1, i.nnz), dtype=np.float64)
+        L.check(self._lib.fem_ctx_get_matrix(self.handle, int(which), L.ptr(out, C.c_double)))
+        return out[: i.nnz]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fem_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HIPSolver:
+    """The third backend behind the reference's Solver interface."""
+
+    def __init__(self, fem_ctx: FEMContext, options: Optional[SolverOptions] = None, **kw):
+        self._lib = L.load()
+        self.fem_ctx = fem_ctx
+        self.options = options or SolverOptions(**kw)
+        h = C.c_void_p()
+        o = self.options.to_c()
+        L.check(self._lib.fem_solver_create(fem_ctx.handle, C.byref(o), C.byref(h)))
+        self.handle = h
+        self.last_stats = L.fem_step_stats()
+        self._eq = None
+        self.set_equation()
+
+    # -- parameters -------------------------------------------------------
+    def set_equation(self, equation: Optional[int] = None, params=None):
+        if equation is None:
+            equation = self.fem_ctx.equation
+        if params is None:
+            params = self.fem_ctx.parameters[equation]
+        eq, cp = params.to_c()
+        if eq != equation:
+            raise ValueError("parameter class does not match the equation")
+        L.check(self._lib.fem_solver_set_equation(self.handle, eq, C.byref(cp), C.sizeof(cp)))
+        self._eq = equation
+
+    # -- Solver interface -------------------------------------------------
+    def advance_time(self, nsteps: int = 1) -> int:
+        """Solver::advance_time; returns FEM_OK / FEM_NOT_CONVERGED / FEM_DIVERGED."""
+        self.set_equation(self.fem_ctx.equation)
+        return L.check(self._lib.fem_solver_advance(self.handle, int(nsteps), C.byref(self.last_stats)))
+
+    def advance(self, nsteps: int = 1) -> int:
+        """Advance with the equation already set (no parameter re-upload)."""
+        return L.check(self._lib.fem_solver_advance(self.handle, int(nsteps), C.byref(self.last_stats)))
+
+    def clear_values(self):
+        L.check(self._lib.fem_solver_clear(self.handle))
+
+    def has_numerical_instability(self) -> bool:
+        out = C.c_int32()
+        L.check(self._lib.fem_solver_unstable(self.handle, C.byref(out)))
+        return bool(out.value)
+
+    # -- state --------------------------------------------------------------
+    def _ncols(self, field_id):
+        return 1 if self._eq == Equation.Reaction_Diffusion else self.options.nrhs
+
+    def set_state(self, field_id: int, values_nv):
+        v = np.ascontiguousarray(values_nv, dtype=np.float32).reshape(-1)
+        L.check(self._lib.fem_solver_set_state(self.handle, int(field_id), L.ptr(v, C.c_float)))
+
+    def get_state(self, field_id: int = L.FEM_FIELD_U):
+        nv = self.fem_ctx.num_nodes()
+        out = np.empty(nv * self._ncols(field_id), dtype=np.float32)
+        L.check(self._lib.fem_solver_get_state(self.handle, int(field_id), L.ptr(out, C.c_float)))
+        return out
+
+    def set_state_unknowns(self, field_id: int, x):
+        v = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        L.check(self._lib.fem_solver_set_state_unknowns(self.handle, int(field_id), L.ptr(v, C.c_double)))
+
+    def get_state_unknowns(self, field_id: int = L.FEM_FIELD_U):
+        n = self.fem_ctx.num_unknowns()
+        out = np.empty(n * self._ncols(field_id), dtype=np.float64)
+        L.check(self._lib.fem_solver_get_state_unknowns(self.handle, int(field_id), L.ptr(out, C.c_double)))
+        return out
+
+    def device_ptr(self, field_id: int = L.FEM_FIELD_U) -> int:
+        p = C.c_void_p()
+        L.check(self._lib.fem_solver_state_device_ptr(self.handle, int(field_id), C.byref(p)))
+        return p.value
+
+    def sync(self):
+        L.check(self._lib.fem_solver_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fem_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many contexts advanced together in one persistent launch."""
+
+    def __init__(self, contexts: Sequence[FEMContext], options: Optional[SolverOptions] = None, **kw):
+        self._lib = L.load()
+        self.contexts = list(contexts)
+        self.options = options or SolverOptions(**kw)
+        arr = (C.c_void_p * len(self.contexts))(*[c.handle.value for c in self.contexts])
+        h = C.c_void_p()
+        o = self.options.to_c()
+        L.check(self._lib.fem_batch_create(arr, len(self.contexts), C.byref(o), C.byref(h)))
+        self.handle = h
+        self.last_stats = L.fem_step_stats()
+
+    def set_equation(self, instance: int, params):
+        eq, cp = params.to_c()
+        L.check(self._lib.fem_batch_set_equation(self.handle, int(instance), eq, C.byref(cp), C.sizeof(cp)))
+
+    def set_state_unknowns(self, instance: int, field_id: int, x):
+        v = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        L.check(self._lib.fem_batch_set_state_unknowns(self.handle, int(instance), int(field_id),
+                                                       L.ptr(v, C.c_double)))
+
+    def get_state_unknowns(self, instance: int, field_id: int = L.FEM_FIELD_U):
+        n = self.contexts[instance].num_unknowns()
+        out = np.empty(n * self.options.nrhs, dtype=np.float64)
+        L.check(self._lib.fem_batch_get_state_unknowns(self.handle, int(instance), int(field_id),
+                                                       L.ptr(out, C.c_double)))
+        return out
+
+    def advance(self, nsteps: int = 1) -> int:
+        return L.check(self._lib.fem_batch_advance(self.handle, int(nsteps), C.byref(self.last_stats)))
+
+    def sync(self):
+        L.check(self._lib.fem_batch_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fem_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

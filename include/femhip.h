@@ -1,0 +1,194 @@
+/*
+ * femhip.h -- C ABI of the MI355X (gfx950) FEM time-stepping backend.
+ *
+ * The library assembles P1-triangle stiffness / mass / advection matrices into
+ * CSR on the GPU and advances the heat, wave, advection-diffusion and
+ * Gray-Scott equations with implicit time steps, behind the reference's
+ * solver interface (include/FEM/Solver.hpp:8-15 of tvumcc/fea-visualizer).
+ *
+ * Conventions
+ *   - Every function returns an int status: FEM_OK (0), a positive solver
+ *     outcome (FEM_NOT_CONVERGED, FEM_DIVERGED) or a negative error.  No C++
+ *     exception crosses this boundary; fem_last_error() returns a thread-local
+ *     message for the last failure.
+ *   - Host arrays are owned by the caller and copied in or out.  The library
+ *     owns its device memory and one HIP stream per handle.
+ *   - A handle is used from one host thread at a time (the reference drives
+ *     its solvers from the single GL thread).
+ */
+#ifndef FEMHIP_H
+#define FEMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMHIP_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  FEM_OK = 0,
+  FEM_NOT_CONVERGED = 1,   /* a solve hit its iteration cap */
+  FEM_DIVERGED = 2,        /* instability rule tripped or non-finite state */
+  FEM_ERR_INVALID = -1,    /* bad argument */
+  FEM_ERR_HIP = -2,        /* HIP runtime error (no GPU, launch failure, ...) */
+  FEM_ERR_OOM = -3,
+  FEM_ERR_STATE = -4,      /* call out of order (e.g. advance before set_equation) */
+  FEM_ERR_TIMEOUT = -5     /* a device-side wait exceeded its spin limit */
+};
+
+/* Equation ids: 0-3 as enum class Equation (include/FEM/FEMContext.hpp:9-14). */
+enum {
+  FEM_HEAT = 0,
+  FEM_WAVE = 1,                  /* the reference's velocity-form backward Euler */
+  FEM_ADVECTION_DIFFUSION = 2,
+  FEM_REACTION_DIFFUSION = 3,    /* Gray-Scott */
+  FEM_WAVE_NEWMARK = 4           /* no reference counterpart: Newmark-beta */
+};
+
+/* Boundary conditions: enum class BoundaryCondition (FEMContext.hpp:16-19). */
+enum { FEM_DIRICHLET = 0, FEM_NEUMANN = 1 };
+
+/* State fields. */
+enum { FEM_FIELD_U = 0, FEM_FIELD_V = 1, FEM_FIELD_A = 2 };
+
+/* Matrices held by a context. */
+enum { FEM_MATRIX_K = 0, FEM_MATRIX_M = 1, FEM_MATRIX_A = 2 };
+
+/* Solver engines and arithmetic. */
+enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1 };
+enum { FEM_PREC_FP64 = 0, FEM_PREC_FP32_IR = 1 };
+enum { FEM_KRYLOV_AUTO = 0, FEM_KRYLOV_CG = 1, FEM_KRYLOV_BICGSTAB = 2 };
+/* Gray-Scott reaction weighting: the reference adds the nodal reaction
+ * without mass weighting (CPUSolver.cpp:134,137); LUMPED multiplies it by the
+ * row sums of M (a documented deviation that is stable at large time steps). */
+enum { FEM_REACTION_NODAL = 0, FEM_REACTION_LUMPED = 1 };
+
+/* Mesh: Surface::{vertices, triangles, on_boundary} (Surface.hpp:36-40). */
+typedef struct fem_mesh {
+  int32_t num_vertices;
+  int32_t num_triangles;
+  const float* xyz;            /* 3 * num_vertices */
+  const uint32_t* triangles;   /* 3 * num_triangles */
+  const uint8_t* on_boundary;  /* num_vertices, 0/1 */
+} fem_mesh;
+
+/* Assembly inputs: AdvectionDiffusionParameters::velocity
+ * (EquationParameters.hpp:17-26, read at FEMContext.cpp:58) plus extensions. */
+typedef struct fem_assembly_params {
+  double velocity[3];          /* constant velocity */
+  const double* velocity_e;    /* optional: 3 per triangle (overrides velocity) */
+  int32_t normalize_velocity;  /* 1 = reference (unit tangential velocity, FEMContext.cpp:180) */
+  const double* kappa_e;       /* optional: per-triangle diffusivity multiplying K */
+} fem_assembly_params;
+
+/* Equation parameters: EquationParameters.hpp:4-49 (widened to double). */
+typedef struct fem_heat_params { double time_step; double conductivity; } fem_heat_params;
+typedef struct fem_advdiff_params { double time_step; double c; } fem_advdiff_params;
+typedef struct fem_wave_params { double time_step; double c; } fem_wave_params;
+typedef struct fem_rd_params {
+  double time_step;
+  double Du;
+  double Dv;
+  double feed_rate;
+  double kill_rate;
+  int32_t reaction_weighting;  /* FEM_REACTION_NODAL (reference) or FEM_REACTION_LUMPED */
+} fem_rd_params;
+typedef struct fem_newmark_params { double time_step; double c; double beta; double gamma; } fem_newmark_params;
+
+typedef struct fem_solver_opts {
+  int32_t engine;        /* FEM_ENGINE_PERSISTENT (default) or FEM_ENGINE_PLAIN (FP64 baseline) */
+  int32_t precision;     /* FEM_PREC_FP32_IR (default) or FEM_PREC_FP64 */
+  int32_t krylov;        /* FEM_KRYLOV_AUTO: CG, BiCGSTAB for advection-diffusion */
+  int32_t nrhs;          /* right-hand sides per instance, 1..16 (Gray-Scott: 1) */
+  int32_t warm_start;    /* 1: start from the previous state; 0: x0 = 0 as Eigen's solve() */
+  int32_t max_iters;     /* Krylov iterations per solve; 0 = 2N (Eigen's default) */
+  int32_t max_inner;     /* FP32 iterations per refinement sweep; 0 = 200 */
+  int32_t max_sweeps;    /* refinement sweeps per solve; 0 = 30 */
+  int32_t team_size;     /* workgroups per instance; 0 = autotuned */
+  double tol;            /* relative residual |b - Ax| <= tol |b| in FP64; 0 = 1e-10 */
+  double inner_tol;      /* FP32 sweep reduction; 0 = 1e-4 */
+} fem_solver_opts;
+
+typedef struct fem_step_stats {
+  int64_t iterations;    /* Krylov iterations summed over solves and steps */
+  int64_t sweeps;        /* FP64 refinement sweeps */
+  int64_t steps;
+  double residual;       /* largest final relative FP64 residual of the last step */
+  double ms;             /* device time of the call */
+  int32_t status;        /* OR of the per-instance outcome bits */
+} fem_step_stats;
+
+typedef struct fem_ctx_info {
+  int32_t num_nodes;        /* FEMContext::num_nodes */
+  int32_t num_unknowns;     /* FEMContext::num_unknowns */
+  int32_t num_triangles;
+  int32_t max_row_nonzeros; /* FEMContext::num_max_nonzeros_per_row */
+  int64_t nnz;
+} fem_ctx_info;
+
+typedef struct fem_ctx fem_ctx;
+typedef struct fem_solver fem_solver;
+typedef struct fem_batch fem_batch;
+typedef struct fem_adjoint fem_adjoint;
+
+const char* fem_last_error(void);
+int fem_abi_version(void);
+/* Number of visible GPUs (0 when none); never throws. */
+int fem_device_count(int32_t* count);
+void fem_opts_default(fem_solver_opts* opts);
+
+/* ---- context: mesh, idx_map, CSR pattern, assembled matrices ----
+ * fem_ctx_create replaces FEMContext::init_from_surface +
+ * update_boundary_conditions (FEMContext.cpp:21-27, 224-240);
+ * fem_ctx_assemble replaces FEMContext::assemble_matrices
+ * (FEMContext.cpp:57-64, 70-219).  create also assembles with the reference
+ * defaults (velocity (1,0,0), normalised). */
+int fem_ctx_create(const fem_mesh* mesh, int32_t bc, fem_ctx** out);
+int fem_ctx_destroy(fem_ctx* ctx);
+int fem_ctx_assemble(fem_ctx* ctx, const fem_assembly_params* params);
+/* FEMContext::num_nodes / num_unknowns / num_max_nonzeros_per_row (FEMContext.cpp:33-52). */
+int fem_ctx_info_get(const fem_ctx* ctx, fem_ctx_info* info);
+int fem_ctx_get_idx_map(const fem_ctx* ctx, int32_t* idx_map);
+int fem_ctx_get_pattern(const fem_ctx* ctx, int32_t* row_ptr, int32_t* col_idx);
+int fem_ctx_get_matrix(const fem_ctx* ctx, int32_t which, double* values);
+
+/* ---- solver: the Solver interface (Solver.hpp:8-15) ----
+ * fem_solver_advance  <- Solver::advance_time        (CPUSolver.cpp:64-147)
+ * fem_solver_clear    <- Solver::clear_values        (CPUSolver.cpp:54-59)
+ * fem_solver_unstable <- Solver::has_numerical_instability (CPUSolver.cpp:41-48)
+ * set/get_state work in surface space (nv values, gather/scatter through
+ * idx_map as CPUSolver.cpp:13-37; boundary values read back as 0).  With
+ * nrhs > 1 the arrays hold nv*nrhs values, right-hand side fastest. */
+int fem_solver_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_solver** out);
+int fem_solver_destroy(fem_solver* s);
+int fem_solver_set_equation(fem_solver* s, int32_t equation, const void* params, size_t size);
+int fem_solver_set_state(fem_solver* s, int32_t field, const float* values_nv);
+int fem_solver_get_state(fem_solver* s, int32_t field, float* values_nv);
+int fem_solver_set_state_unknowns(fem_solver* s, int32_t field, const double* x);
+int fem_solver_get_state_unknowns(fem_solver* s, int32_t field, double* x);
+int fem_solver_advance(fem_solver* s, int32_t nsteps, fem_step_stats* stats);
+int fem_solver_clear(fem_solver* s);
+int fem_solver_unstable(fem_solver* s, int32_t* unstable);
+/* Device pointer of a state field ([N][nrhs] doubles) for zero-copy callers. */
+int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr);
+/* Synchronise the solver stream. */
+int fem_solver_sync(fem_solver* s);
+
+/* ---- batch: many contexts (meshes of uneven size) advanced in one
+ * persistent launch; no reference counterpart ---- */
+int fem_batch_create(fem_ctx* const* ctxs, int32_t count, const fem_solver_opts* opts, fem_batch** out);
+int fem_batch_destroy(fem_batch* b);
+int fem_batch_set_equation(fem_batch* b, int32_t instance, int32_t equation, const void* params, size_t size);
+int fem_batch_set_state_unknowns(fem_batch* b, int32_t instance, int32_t field, const double* x);
+int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, double* x);
+int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats);
+int fem_batch_sync(fem_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEMHIP_H */

@@ -1,0 +1,33 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "fea-visualizer_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+def _has_gpu():
+    try:
+        import ctypes
+        from femhip import _lib
+        lib = _lib.load()
+        c = ctypes.c_int32(0)
+        lib.fem_device_count(ctypes.byref(c))
+        return c.value > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    if not _has_gpu():
+        pytest.fail("GPU test selected but no GPU / library is available")
+    from femhip import _lib
+    return _lib.load()

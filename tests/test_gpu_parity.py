@@ -1,0 +1,247 @@
+"""GPU parity tests: the HIP library (through the C ABI) against the oracle
+and the golden fixtures.  Pattern and idx_map are compared bit for bit;
+FP64 matrices to 1e-12 of their largest entry; every time-step result to
+rel-L2 <= 1e-8 against the FP64 oracle solved to 1e-14 (the tolerance of
+the benchmark's accuracy gate)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from femhip import (AdvectionDiffusionParameters, Batch, Equation, FEMContext, HeatParameters, HIPSolver,
+                    NewmarkParameters, ReactionDiffusionParameters, SolverOptions, WaveParameters, _lib, meshes)
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+RTOL = 1e-8
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0)
+
+
+def gauss(mesh, c=(0.5, 0.5), w=0.01):
+    x = mesh["xyz"].astype(np.float64)
+    return np.exp(-((x[:, 0] - c[0]) ** 2 + (x[:, 1] - c[1]) ** 2) / w)
+
+
+MESHES = {
+    "square16": (lambda: meshes.structured_square(16), 0),
+    "jitter20": (lambda: meshes.jittered_square(20, seed=4), 0),
+    "jitter20N": (lambda: meshes.jittered_square(20, seed=4), 1),
+    "ico3": (lambda: meshes.icosphere(3), 1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_pattern_and_matrices(gpu, name):
+    mk, bc = MESHES[name]
+    m = mk()
+    print("case", name, flush=True)
+    ctx = FEMContext(m, bc, velocity=(0.3, 1.0, 0.2))
+    s = oracle.System(m, bc=bc, velocity=(0.3, 1.0, 0.2))
+    assert np.array_equal(ctx.idx_map, s.idx_map)
+    rp, ci = ctx.pattern()
+    assert np.array_equal(rp, s.row_ptr)
+    assert np.array_equal(ci, s.col_idx)
+    info = ctx.info()
+    assert info.num_unknowns == s.n and info.nnz == s.nnz
+    assert info.max_row_nonzeros == int(np.diff(s.row_ptr).max())
+    for which, ref in ((_lib.FEM_MATRIX_K, s.K), (_lib.FEM_MATRIX_M, s.M), (_lib.FEM_MATRIX_A, s.A)):
+        got = ctx.matrix(which)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1e-300, np.max(np.abs(ref)))
+
+
+def test_golden_square_matrices(gpu):
+    g = dict(np.load(os.path.join(GOLD, "square64.npz")))
+    m = {"xyz": g["xyz"], "tri": g["tri"], "on_boundary": g["on_boundary"]}
+    ctx = FEMContext(m, 0)
+    rp, ci = ctx.pattern()
+    assert np.array_equal(ctx.idx_map, g["idx_map"])
+    assert np.array_equal(rp, g["row_ptr"]) and np.array_equal(ci, g["col_idx"])
+    for which, key in ((_lib.FEM_MATRIX_K, "K"), (_lib.FEM_MATRIX_M, "M"), (_lib.FEM_MATRIX_A, "A")):
+        assert np.max(np.abs(ctx.matrix(which) - g[key])) <= 1e-12 * np.max(np.abs(g[key]))
+
+
+def make_solver(ctx, eq, params, **opts):
+    ctx.equation = eq
+    ctx.parameters[eq] = params
+    return HIPSolver(ctx, SolverOptions(**opts))
+
+
+@pytest.mark.parametrize("precision,engine", [
+    (_lib.FEM_PREC_FP32_IR, _lib.FEM_ENGINE_PERSISTENT),
+    (_lib.FEM_PREC_FP64, _lib.FEM_ENGINE_PERSISTENT),
+    (_lib.FEM_PREC_FP64, _lib.FEM_ENGINE_PLAIN),
+])
+def test_heat_golden_steps(gpu, precision, engine):
+    """64x64 square, dt=1e-3, c=1: three steps against the direct-solve golden."""
+    g = dict(np.load(os.path.join(GOLD, "square64.npz")))
+    m = {"xyz": g["xyz"], "tri": g["tri"], "on_boundary": g["on_boundary"]}
+    print("case heat golden", precision, engine, flush=True)
+    ctx = FEMContext(m, 0)
+    sol = make_solver(ctx, Equation.Heat, HeatParameters(1e-3, 1.0), precision=precision, engine=engine, tol=1e-12)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, g["u0"])
+    for k in range(3):
+        st = sol.advance_time(1)
+        assert st == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(), g["heat"][k]) < RTOL
+
+
+@pytest.mark.parametrize("precision", [_lib.FEM_PREC_FP32_IR, _lib.FEM_PREC_FP64])
+def test_heat_trajectory_vs_oracle(gpu, precision):
+    m = meshes.jittered_square(40, seed=1)
+    s = oracle.System(m, bc=0)
+    u0 = s.gather(gauss(m))
+    ctx = FEMContext(m, 0)
+    sol = make_solver(ctx, Equation.Heat, HeatParameters(1e-3, 1.0), precision=precision)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    u = u0.copy()
+    for k in range(10):
+        _, u, _, _ = s.step(oracle.HEAT, [1e-3, 1.0], u, tol=1e-14)
+        sol.advance_time(1)
+        assert rel(sol.get_state_unknowns(), u) < RTOL, k
+    # many steps in one launch
+    _, u, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], 20, u, tol=1e-14)
+    assert sol.advance_time(20) == _lib.FEM_OK
+    assert rel(sol.get_state_unknowns(), u) < RTOL
+    assert sol.last_stats.steps == 20 and sol.last_stats.iterations > 0
+
+
+@pytest.mark.parametrize("engine", [_lib.FEM_ENGINE_PERSISTENT, _lib.FEM_ENGINE_PLAIN])
+def test_advection_diffusion_vs_oracle(gpu, engine):
+    m = meshes.jittered_square(32, seed=2)
+    vel = (0.6, 0.8, 0.0)
+    s = oracle.System(m, bc=0, velocity=vel)
+    ctx = FEMContext(m, 0, velocity=vel)
+    p = AdvectionDiffusionParameters(1e-3, 0.25, vel)
+    sol = make_solver(ctx, Equation.Advection_Diffusion, p, engine=engine,
+                      precision=_lib.FEM_PREC_FP32_IR if engine == 0 else _lib.FEM_PREC_FP64)
+    u = s.gather(gauss(m, (0.4, 0.45), 0.02))
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    for k in range(5):
+        _, u, _, _ = s.step(oracle.ADVDIFF, [1e-3, 0.25], u, krylov=oracle.BICGSTAB, tol=1e-14)
+        assert sol.advance_time(1) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(), u) < RTOL, k
+
+
+def test_wave_reference_scheme_vs_golden(gpu):
+    g = dict(np.load(os.path.join(GOLD, "square64.npz")))
+    m = {"xyz": g["xyz"], "tri": g["tri"], "on_boundary": g["on_boundary"]}
+    ctx = FEMContext(m, 0)
+    sol = make_solver(ctx, Equation.Wave, WaveParameters(0.05, 0.05))
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, g["u0"])
+    for k in range(3):
+        assert sol.advance_time(1) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), g["wave_u"][k]) < RTOL
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), g["wave_v"][k]) < RTOL
+
+
+def test_grayscott_nodal_shipped_icosphere_golden(gpu):
+    """Reference defaults on the reference's own icosphere.obj (u starts at 0)."""
+    g = dict(np.load(os.path.join(GOLD, "icosphere_shipped.npz")))
+    m = {"xyz": g["xyz"], "tri": g["tri"], "on_boundary": g["on_boundary"]}
+    ctx = FEMContext(m, 1)
+    sol = make_solver(ctx, Equation.Reaction_Diffusion, ReactionDiffusionParameters())
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, g["v0"])
+    for k in range(3):
+        assert sol.advance_time(1) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), g["gs_u"][k]) < RTOL
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), g["gs_v"][k]) < RTOL
+
+
+@pytest.mark.parametrize("precision", [_lib.FEM_PREC_FP32_IR, _lib.FEM_PREC_FP64])
+def test_grayscott_lumped_vs_oracle(gpu, precision):
+    m = meshes.icosphere(3)
+    s = oracle.System(m, bc=1)
+    rng = np.random.default_rng(0)
+    u = np.ones(s.n)
+    v = np.zeros(s.n)
+    idx = rng.choice(s.n, 30, replace=False)
+    v[idx] = 0.25 + rng.uniform(0, 0.05, 30)
+    prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    ctx = FEMContext(m, 1)
+    sol = make_solver(ctx, Equation.Reaction_Diffusion, prm, precision=precision)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, v)
+    op = [0.5, 2e-4, 1e-4, 0.037, 0.06, 1]
+    for k in range(20):
+        _, u, v, _ = s.step(oracle.GRAYSCOTT, op, u, v, tol=1e-14)
+    assert sol.advance_time(20) == _lib.FEM_OK
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+
+
+def test_newmark_vs_oracle_and_energy(gpu):
+    m = meshes.jittered_square(32, seed=6)
+    s = oracle.System(m, bc=0)
+    u = s.gather(np.exp(-((m["xyz"][:, 0] - 0.5) ** 2 + (m["xyz"][:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2)))
+    v = np.zeros(s.n)
+    a = s.newmark_init(1.0, u)
+    ctx = FEMContext(m, 0)
+    sol = make_solver(ctx, Equation.Wave_Newmark, NewmarkParameters(2e-3, 1.0, 0.25, 0.5))
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    sol.set_state_unknowns(_lib.FEM_FIELD_A, a)
+    for _ in range(15):
+        _, u, v, a = s.step(oracle.NEWMARK, [2e-3, 1.0, 0.25, 0.5], u, v, a, tol=1e-14)
+    assert sol.advance_time(15) == _lib.FEM_OK
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+
+
+def test_batch_uneven_meshes_multi_rhs(gpu):
+    specs = [(12, 1), (20, 2), (7, 3)]
+    ctxs, systems, states = [], [], []
+    rng = np.random.default_rng(5)
+    k = 4
+    for side, seed in specs:
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0))
+        s = oracle.System(m, bc=0)
+        systems.append(s)
+        states.append(rng.normal(size=(s.n, k)))
+    b = Batch(ctxs, SolverOptions(nrhs=k))
+    kappas = [1e-1, 1e-2, 3e-3]
+    for i, kap in enumerate(kappas):
+        b.set_equation(i, HeatParameters(1e-3, kap))
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+    assert b.advance(3) == _lib.FEM_OK
+    for i, s in enumerate(systems):
+        got = b.get_state_unknowns(i).reshape(s.n, k)
+        for j in range(k):
+            _, u, _, _ = s.run(oracle.HEAT, [1e-3, kappas[i]], 3, states[i][:, j].copy(), tol=1e-14)
+            assert rel(got[:, j], u) < RTOL, (i, j)
+
+
+def test_surface_state_and_instability(gpu):
+    m = meshes.structured_square(16)
+    ctx = FEMContext(m, 0)
+    sol = make_solver(ctx, Equation.Heat, HeatParameters(1e-3, 1.0))
+    vals = gauss(m).astype(np.float32)
+    sol.set_state(_lib.FEM_FIELD_U, vals)
+    got = sol.get_state(_lib.FEM_FIELD_U)
+    onb = m["on_boundary"].astype(bool)
+    assert np.all(got[onb] == 0.0)  # CPUSolver::map_vector_to_surface writes 0 on the boundary
+    assert np.array_equal(got[~onb], vals[~onb])
+    assert sol.advance_time(1) == _lib.FEM_OK and not sol.has_numerical_instability()
+    vals[40] = 5e6  # the reference rule: u > 1e4
+    sol.set_state(_lib.FEM_FIELD_U, vals)
+    assert sol.advance_time(1) == _lib.FEM_DIVERGED
+    assert sol.has_numerical_instability()
+    sol.clear_values()
+    assert not np.any(sol.get_state(_lib.FEM_FIELD_U))
+
+
+def test_deterministic_bitwise(gpu):
+    m = meshes.jittered_square(48, seed=3)
+    out = []
+    for _ in range(2):
+        ctx = FEMContext(m, 0)
+        sol = make_solver(ctx, Equation.Heat, HeatParameters(1e-3, 1.0))
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, np.linspace(0, 1, ctx.num_unknowns()))
+        sol.advance_time(5)
+        out.append(sol.get_state_unknowns())
+    assert np.array_equal(out[0], out[1])
