@@ -47,9 +47,13 @@ __global__ void k_form(int n, const int* rp, const int* diag, const double* M, c
   }
 }
 
-__global__ void k_reset(TeamBar* bars, int nteams, int* abort_flag, Inst* insts, int ninst) {
+// Barrier epochs restart at 0 every launch.  (A fill kernel rather than a
+// memset node, so the launch sequence can be captured in a graph.)
+__global__ void k_reset(Slot* slots, int nslots, TeamGen* gens, int nteams, int* abort_flag, Inst* insts,
+                        int ninst) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nteams) bars[i].count = 0u;
+  if (i < nslots) slots[i].epoch = 0u;
+  if (i < nteams) gens[i].gen = 0u;
   if (i == 0) *abort_flag = 0;
   if (i < ninst) {
     insts[i].counters[0] = 0;
@@ -183,7 +187,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.eq = h.eq;
   int kr = o.krylov;
   if (kr == FEM_KRYLOV_AUTO) kr = (h.eq == FEM_ADVECTION_DIFFUSION) ? FEM_KRYLOV_BICGSTAB : FEM_KRYLOV_CG;
-  I.krylov = (kr == FEM_KRYLOV_BICGSTAB) ? 1 : 0;
+  I.krylov = (kr == FEM_KRYLOV_BICGSTAB) ? 1 : (kr == FEM_KRYLOV_CG_CLASSIC) ? 2 : 0;
   I.warm = o.warm_start;
   I.max_inner = o.max_inner > 0 ? o.max_inner : 200;
   I.max_sweeps = o.max_sweeps > 0 ? o.max_sweeps : 30;
@@ -222,15 +226,15 @@ void fill_dev(fem_batch* b, InstHost& h) {
 }
 
 void free_plan(fem_batch* b) {
-  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_bars, b->d_partial, b->d_scal, b->d_abort};
+  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_slots, b->d_gens, b->d_scal, b->d_abort};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b->d_insts = nullptr;
   b->d_teams = nullptr;
   b->d_wg_team = nullptr;
   b->d_team_inst = nullptr;
-  b->d_bars = nullptr;
-  b->d_partial = nullptr;
+  b->d_slots = nullptr;
+  b->d_gens = nullptr;
   b->d_scal = nullptr;
   b->d_abort = nullptr;
   b->plan_ready = false;
@@ -250,9 +254,15 @@ int build_plan(fem_batch* b) {
       set_error("batch instances must share the number of right-hand side columns");
       return FEM_ERR_INVALID;
     }
+  const int kr = b->insts[0].dev.krylov;
+  for (auto& h : b->insts)
+    if (h.dev.krylov != kr) {
+      set_error("batch instances must share the Krylov method (set opts.krylov explicitly)");
+      return FEM_ERR_INVALID;
+    }
   const int inner_fp32 = inner_bytes(b) == 4;
   int maxb = 0;
-  int rc = step_max_coresident(k, inner_fp32, &maxb);
+  int rc = step_max_coresident(k, inner_fp32, kr, 1, &maxb);
   if (rc) return rc;
   if (maxb <= 0) {
     set_error("persistent kernel cannot be resident on this device");
@@ -273,8 +283,8 @@ int build_plan(fem_batch* b) {
     }
   } else if (ninst == 1) {
     const long long lanes = (long long)b->insts[0].ctx->n * k;
-    int g = (int)std::max(1LL, (lanes + 4 * kBlock - 1) / (4 * kBlock));
-    g = std::min(g, maxb);
+    int g = (int)std::max(1LL, (lanes + kBlock - 1) / kBlock);  // ~one row per lane
+    g = std::min(g, std::min(maxb, 1024));
     teams.push_back({0});
     sizes.push_back(g);
   } else {
@@ -336,22 +346,26 @@ int build_plan(fem_batch* b) {
   b->d_teams = dalloc<Team>(nteams, &rc);
   b->d_wg_team = dalloc<int>(grid, &rc);
   b->d_team_inst = dalloc<int>(team_inst.size(), &rc);
-  b->d_bars = dalloc<TeamBar>(nteams, &rc);
-  b->d_partial = dalloc<double>((size_t)grid * kMaxRed, &rc);
-  b->d_scal = dalloc<double>((size_t)nteams * kMaxRed, &rc);
+  b->d_slots = dalloc<Slot>(grid, &rc);
+  b->d_gens = dalloc<TeamGen>(nteams, &rc);
+  b->d_scal = dalloc<double>((size_t)nteams * 2 * kMaxRed, &rc);
   b->d_abort = dalloc<int>(1, &rc);
   if (rc) return rc;
   int poff = 0;
   for (int t = 0; t < nteams; t++) {
-    ht[t].bar = b->d_bars + t;
-    ht[t].partial = b->d_partial + (size_t)poff * kMaxRed;
-    ht[t].scal = b->d_scal + (size_t)t * kMaxRed;
+    ht[t].slots = b->d_slots + poff;
+    ht[t].gen = b->d_gens + t;
+    ht[t].scal = b->d_scal + (size_t)t * 2 * kMaxRed;
     ht[t].abort_flag = b->d_abort;
+    // every workgroup reading every partial costs size^2 * nvals loads per
+    // barrier; beyond ~16 KB per workgroup rank 0 reduces and publishes
+    ht[t].allpoll = ((long long)sizes[t] * 3 * k <= 2048) ? 1 : 0;
     poff += sizes[t];
   }
   std::vector<Inst> hi(ninst);
   for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
-  FEM_HIP(hipMemsetAsync(b->d_bars, 0, sizeof(TeamBar) * nteams, b->stream));
+  FEM_HIP(hipMemsetAsync(b->d_slots, 0, sizeof(Slot) * grid, b->stream));
+  FEM_HIP(hipMemsetAsync(b->d_gens, 0, sizeof(TeamGen) * nteams, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_teams, ht.data(), sizeof(Team) * nteams, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_wg_team, wg_team.data(), sizeof(int) * grid, hipMemcpyHostToDevice, b->stream));
@@ -365,7 +379,11 @@ int build_plan(fem_batch* b) {
   b->launch.grid = grid;
   b->launch.k = k;
   b->launch.inner_fp32 = inner_fp32;
+  b->launch.kr = kr;
+  b->launch.coh = 0;
+  for (int g : sizes) b->launch.coh |= (g > 1) ? 1 : 0;
   b->nteams = nteams;
+  b->grid = grid;
   b->team_sizes = sizes;
   b->plan_ready = true;
   return 0;
@@ -395,9 +413,9 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
   if (opts) b->opts = *opts;
   if (b->opts.nrhs <= 0) b->opts.nrhs = 1;
   const int nr = b->opts.nrhs;
-  if (nr > kMaxK || (nr & (nr - 1)) != 0) {
+  if (nr != 1 && nr != 2 && nr != 4 && nr != 16) {
     delete b;
-    set_error("nrhs must be 1, 2, 4, 8 or 16");
+    set_error("nrhs must be 1, 2, 4 or 16");
     return FEM_ERR_INVALID;
   }
   if (b->opts.precision != FEM_PREC_FP64 && b->opts.precision != FEM_PREC_FP32_IR) {
@@ -651,8 +669,8 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   int rc = batch_prepare(b, true);
   if (rc) return rc;
   const int ninst = (int)b->insts.size();
-  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(b->nteams, ninst), kBlock)), dim3(kBlock), 0, b->stream,
-                     b->d_bars, b->nteams, b->d_abort, b->d_insts, ninst);
+  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(std::max(b->nteams, ninst), b->grid), kBlock)), dim3(kBlock), 0,
+                     b->stream, b->d_slots, b->grid, b->d_gens, b->nteams, b->d_abort, b->d_insts, ninst);
   FEM_HIP(hipGetLastError());
   FEM_HIP(hipEventRecord(b->ev0, b->stream));
   if (nsteps > 0) {

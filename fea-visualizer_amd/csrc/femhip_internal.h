@@ -37,12 +37,20 @@ enum : int {
   ST_BARRIER_TIMEOUT = 8,
 };
 
-// Team barrier state: one per team, 128-byte separated counters.
-struct alignas(128) TeamBar {
-  unsigned int count;
-  unsigned int pad0[31];
+// Per-workgroup barrier slot: the partial sums of the last two barriers
+// (double-buffered by epoch parity) and the epoch this workgroup reached.
+// Arrival is a plain release store to the workgroup's own slot, so arrivals
+// never contend on one address.
+struct alignas(128) Slot {
+  double vals[2][kMaxRed];
+  unsigned int epoch;
+  unsigned int pad[31];
+};
+
+// Release word of a team (master-reduction mode) on its own cache line.
+struct alignas(128) TeamGen {
   unsigned int gen;
-  unsigned int pad1[31];
+  unsigned int pad[31];
 };
 
 // Device-side description of one instance (a mesh, its parameters and its
@@ -91,9 +99,12 @@ struct Inst {
 struct Team {
   int first_wg, size;
   int inst_begin, inst_end;  // range in the team-instance list
-  TeamBar* bar;
-  double* partial;           // [size][kMaxRed]
-  double* scal;              // [kMaxRed] reduced values of the last barrier
+  int allpoll;               // 1: every workgroup polls all slots and reduces;
+                             // 0: rank 0 reduces and releases through gen
+  int pad_;
+  Slot* slots;               // [size]
+  TeamGen* gen;
+  double* scal;              // [2][kMaxRed] results published by rank 0
   int* abort_flag;
 };
 

@@ -14,10 +14,12 @@ struct StepLaunch {
   int grid = 0;
   int k = 1;
   int inner_fp32 = 1;
+  int kr = 0;   // Krylov method: 0 pipelined CG, 1 BiCGSTAB, 2 classic CG
+  int coh = 1;  // 1 when any team spans more than one workgroup
 };
 
 int launch_step(const StepLaunch& L, int nsteps, hipStream_t st);
-int step_max_coresident(int k, int inner_fp32, int* blocks);
+int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
 
 // Per-instance host bookkeeping.
 struct InstHost {
@@ -60,9 +62,10 @@ struct fem_batch {
   femhip::Team* d_teams = nullptr;
   int* d_wg_team = nullptr;
   int* d_team_inst = nullptr;
-  femhip::TeamBar* d_bars = nullptr;
-  double* d_partial = nullptr;
+  femhip::Slot* d_slots = nullptr;
+  femhip::TeamGen* d_gens = nullptr;
   double* d_scal = nullptr;
+  int grid = 0;
   int* d_abort = nullptr;
   int nteams = 0;
   std::vector<int> team_sizes;  // for reporting

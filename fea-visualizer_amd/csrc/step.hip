@@ -3,22 +3,39 @@
 // One launch advances every instance of a batch by `nsteps` implicit time
 // steps.  The grid is split into teams (contiguous workgroup ranges); a team
 // owns a list of instances and walks them one after the other.  Inside a team
-// each workgroup owns a contiguous block of CSR rows and the team meets at a
-// device-scope barrier between phases.  The last workgroup to arrive reduces
-// the per-workgroup partial dot products in rank order (so every reduction is
-// deterministic) and publishes the result with the barrier release.
+// each workgroup owns a contiguous block of CSR rows, and the team meets at a
+// barrier between phases.
+//
+// Memory model.  On gfx950 a device-scope fence is a whole-L2 write-back plus
+// invalidate (buffer_wbl2 sc1 / buffer_inv sc1) that the L2 serialises across
+// workgroups, so a fenced barrier costs ~35 ns per participating workgroup.
+// This kernel uses no fences.  In teams of more than one workgroup, every
+// value that one workgroup writes during the launch and another reads later
+// (state, solution, residual, Krylov vectors, partial sums) is stored and
+// loaded device-coherently (agent-scope atomic accesses: global_load/store
+// sc1).  A barrier is then: every wave waits for its stores to complete
+// (s_waitcnt 0), s_barrier, and thread 0 publishes the workgroup's epoch in
+// its own slot with a coherent store -- the release the memory model would
+// emit for such data, without the cache-wide write-back.  Arrivals go to
+// distinct slots (no contended read-modify-write).  Read-only data (pattern,
+// matrices, preconditioner) uses ordinary cached loads.  A one-workgroup team
+// needs none of this and runs the plain variant.
+//
+// Reductions: partial sums are double-buffered by epoch parity in the slots
+// and always summed in rank order, so results are deterministic.
 //
 // Per step and instance (the reference's CPUSolver::advance_time,
 // src/FEM/CPUSolver.cpp:64-147, re-expressed as fused phases):
 //   R    right-hand side (FP64) incl. the Gray-Scott reaction, the initial
-//        residual r = b - S x0 (FP64), r -> inner precision, z = D^-1 r
-//   A/B  preconditioned CG in the inner precision (FP32 by default):
-//        A: p = z + beta p (double-buffered, recomputed for gathered
-//           neighbours), q = S p, <p,q>;  B: alpha, xi += alpha p,
-//           r -= alpha q, z = D^-1 r, <r,z>, <r,r>
-//        (BiCGSTAB for advection-diffusion: phases P, S, X)
-//   Res  FP64 refinement residual r = b - S64 (x + xi), restart of the inner
-//        solve when |r| >= tol |b|
+//        residual r = b - S x0 (FP64), r -> inner precision
+//   I    CG start: w = S D^-1 r, <r, D^-1 r>, <D^-1 r, w>
+//   C    one Chronopoulos-Gear PCG iteration per barrier: p, s = S p, xi, r
+//        updated for own rows; w = S D^-1 r_new with the neighbours' r_new
+//        recomputed from the double-buffered r, w, s of the previous
+//        iteration; <r,u>, <u,w>, <r,r>
+//        (classic two-phase CG and BiCGSTAB are also available)
+//   Res  FP64 refinement residual r = b - S64 (x + xi), next sweep while
+//        |r| >= tol |b|
 //   U    state update (heat: u = x; wave: v = x, u += dt v; Gray-Scott: u,v;
 //        Newmark: u, v, a) and the instability rule (CPUSolver.cpp:41-48)
 // Vectors are [rows][k] with the right-hand side fastest, so the k lanes of a
@@ -47,13 +64,36 @@ struct Shared {
   double ithr[kMaxK];      // inner stopping threshold of the sweep
   double bb[kMaxK];
   int active[kMaxK];
-  int flag;
   int uni;                 // uniform control value broadcast
-  unsigned gen;
 };
 
-__device__ inline double ald(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Pointers loaded from the instance table are generic; casting them to the
+// global address space lets the compiler emit global_load/store instead of
+// flat accesses.
+#define GA __attribute__((address_space(1)))
+template <typename T>
+__device__ inline T GA* gp(T* p) { return (T GA*)p; }
+template <typename T>
+__device__ inline const T GA* gp(const T* p) { return (const T GA*)p; }
+
+// Device-coherent (C = true) or plain (C = false) access to data exchanged
+// between the workgroups of a team.
+template <bool C, typename T>
+__device__ inline T ld(const T* p) {
+  if constexpr (C) return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *gp(p);
+}
+template <bool C, typename T>
+__device__ inline void st(T* p, T v) {
+  if constexpr (C) __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *gp(p) = v;
+}
+
+__device__ inline unsigned uld(const unsigned* p) {
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void ust(unsigned* p, unsigned v) {
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Rank of this workgroup inside its team, ordered so that ranks on the same
@@ -65,8 +105,8 @@ __device__ inline int xcd_rank(int l, int g) {
   return base + (l >> 3);
 }
 
-// Per-lane partial sums -> this workgroup's row of the team partial table.
-template <int K, int NS>
+// Per-lane partial sums -> dst[q*K + j] (this workgroup's share).
+template <int K, int NS, bool C>
 __device__ inline void block_partials(const double (&acc)[NS], Shared& sh, double* dst) {
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
@@ -81,86 +121,186 @@ __device__ inline void block_partials(const double (&acc)[NS], Shared& sh, doubl
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kBlock / kWave; w++) s += sh.wred[w][threadIdx.x];
-    dst[threadIdx.x] = s;
+    if constexpr (C) st<true>(dst + threadIdx.x, s);  // the team's slot in global memory
+    else dst[threadIdx.x] = s;                         // sh.red in LDS
   }
 }
 
-// Team barrier.  nvals > 0: the partial table is reduced and sh.red filled.
-// Returns false when the launch must stop (a wait timed out somewhere).
-__device__ bool team_sync(const Team& T, int nvals, Shared& sh) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned g = __hip_atomic_load(&T.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = __hip_atomic_fetch_add(&T.bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    sh.flag = (prev == (unsigned)(T.size - 1)) ? 1 : 0;
-    sh.gen = g;
-  }
-  __syncthreads();
-  if (sh.flag == 1) {
-    if (nvals > 0) {
-      // chunks of ranks per value, summed in fixed rank order
-      const int chunks = kBlock / nvals;
-      const int q = threadIdx.x % nvals, c = threadIdx.x / nvals;
-      double s = 0.0;
-      if (c < chunks)
-        for (int w = c; w < T.size; w += chunks) s += ald(T.partial + (size_t)w * kMaxRed + q);
-      __syncthreads();
-      double* buf = &sh.wred[0][0];  // kBlock doubles of scratch
-      if (c < chunks) buf[c * nvals + q] = s;
-      __syncthreads();
-      if (threadIdx.x < nvals) {
-        double t = 0.0;
-        for (int cc = 0; cc < chunks; cc++) t += buf[cc * nvals + threadIdx.x];
-        T.scal[threadIdx.x] = t;
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      __hip_atomic_store(&T.bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      __hip_atomic_store(&T.bar->gen, sh.gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  } else if (threadIdx.x == 0) {
-    long long t0 = wall_clock64();
-    while (__hip_atomic_load(&T.bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == sh.gen) {
+__device__ inline bool aborted(const Team& T) {
+  return __hip_atomic_load(T.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// Wait until every slot of the team reached epoch e.  Returns nonzero (on
+// every thread) when the wait timed out or another workgroup aborted.
+__device__ int wait_all(const Team& T, unsigned e) {
+  int to = 0;
+  for (int t = threadIdx.x; t < T.size && !to; t += kBlock) {
+    if (uld(&T.slots[t].epoch) >= e) continue;
+    const long long t0 = wall_clock64();
+    int spins = 0;
+    while (uld(&T.slots[t].epoch) < e) {
       __builtin_amdgcn_s_sleep(1);
-      if (__hip_atomic_load(T.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-          wall_clock64() - t0 > kSpinTimeoutTicks) {
-        __hip_atomic_fetch_or(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.flag = -1;
+      if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+        to = 1;
         break;
       }
     }
-    __threadfence();
+  }
+  return __syncthreads_or(to);
+}
+
+__device__ void raise_abort(const Team& T) {
+  if (threadIdx.x == 0) __hip_atomic_fetch_or(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum slot partials of buffer `buf` over ranks (fixed order) into sh.red.
+__device__ void reduce_slots(const Team& T, int buf, int nvals, Shared& sh, bool publish) {
+  const int chunks = kBlock / nvals;
+  const int q = threadIdx.x % nvals, c = threadIdx.x / nvals;
+  double s = 0.0;
+  if (c < chunks)
+    for (int w = c; w < T.size; w += chunks) s += ld<true>(&T.slots[w].vals[buf][q]);
+  double* scratch = &sh.wred[0][0];  // kBlock doubles
+  if (c < chunks) scratch[c * nvals + q] = s;
+  __syncthreads();
+  if (threadIdx.x < nvals) {
+    double t = 0.0;
+    for (int cc = 0; cc < chunks; cc++) t += scratch[cc * nvals + threadIdx.x];
+    sh.red[threadIdx.x] = t;
+    if (publish) st<true>(&T.scal[buf * kMaxRed + threadIdx.x], t);
+  }
+}
+
+// Team barrier for epoch e; the partials (if nvals > 0) are already in this
+// workgroup's slot buffer e & 1 (or in sh.red for a one-workgroup team).
+// Returns false when the launch must stop.
+__device__ bool team_sync(const Team& T, int rank, unsigned e, int nvals, Shared& sh) {
+  if (T.size == 1) {
+    __syncthreads();
+    return true;
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's coherent stores have completed
+  __syncthreads();
+  const int buf = e & 1;
+  if (threadIdx.x == 0) ust(&T.slots[rank].epoch, e);
+  if (T.allpoll || rank == 0) {
+    if (wait_all(T, e)) {
+      raise_abort(T);
+      return false;
+    }
+    if (nvals > 0) reduce_slots(T, buf, nvals, sh, !T.allpoll);
+    if (!T.allpoll) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0) ust(&T.gen->gen, e);
+    }
+  } else {
+    int to = 0;
+    if (threadIdx.x == 0) {
+      const long long t0 = wall_clock64();
+      int spins = 0;
+      while (uld(&T.gen->gen) < e) {
+        __builtin_amdgcn_s_sleep(1);
+        if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+          to = 1;
+          break;
+        }
+      }
+    }
+    if (__syncthreads_or(to)) {
+      raise_abort(T);
+      return false;
+    }
+    if (threadIdx.x < nvals) sh.red[threadIdx.x] = ld<true>(&T.scal[buf * kMaxRed + threadIdx.x]);
   }
   __syncthreads();
-  if (sh.flag < 0) return false;
-  if (nvals > 0 && threadIdx.x < nvals) sh.red[threadIdx.x] = ald(T.scal + threadIdx.x);
-  __syncthreads();
   return true;
+}
+
+// Reduction + barrier: acc -> team sums in sh.red[q*K + j].
+template <int K, int NS, bool C>
+__device__ bool team_reduce(const double (&acc)[NS], const Team& T, int rank, unsigned& e, Shared& sh) {
+  e++;
+  if constexpr (C) block_partials<K, NS, true>(acc, sh, T.slots[rank].vals[e & 1]);
+  else block_partials<K, NS, false>(acc, sh, sh.red);
+  return team_sync(T, rank, e, NS * K, sh);
+}
+
+__device__ bool team_barrier(const Team& T, int rank, unsigned& e, Shared& sh) {
+  e++;
+  return team_sync(T, rank, e, 0, sh);
 }
 
 template <typename TI>
 __device__ inline TI to_ti(double v) { return (TI)v; }
 
+// Chronopoulos-Gear update of one entry, shared by the owner and by
+// neighbours recomputing it (identical operations, identical rounding).
+template <typename TI>
+__device__ inline TI cg_s(TI w, TI s, TI be, bool first) { return first ? w : (w + be * s); }
+template <typename TI>
+__device__ inline TI cg_r(TI r, TI sv, TI al) { return r - al * sv; }
+
 // ---------------------------------------------------------------------------
 // Phases.  Lane layout: column jr = tid % K, rows tid / K + m * (kBlock / K).
+//
+// Row gathers are unrolled by kU with clamped (always in-bounds) indices, so a
+// row's loads are all in flight before the first is consumed; the masked
+// accumulation keeps the sequential summation order.
 // ---------------------------------------------------------------------------
 
-template <int K, typename TI>
-__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, double (&acc)[3]) {
+constexpr int kU = 8;
+
+template <typename A>
+__device__ inline void acc_add(A& acc, const A& t) { acc += t; }
+
+struct D2 {
+  double a, b;
+  __device__ D2& operator+=(const D2& o) {
+    a += o.a;
+    b += o.b;
+    return *this;
+  }
+};
+struct D3 {
+  double a, b, c;
+  __device__ D3& operator+=(const D3& o) {
+    a += o.a;
+    b += o.b;
+    c += o.c;
+    return *this;
+  }
+};
+
+// sum over s in [lo, hi) of f(s), in ascending s
+template <typename A, typename F>
+__device__ inline A row_gather(int lo, int hi, A zero, F f) {
+  A acc = zero;
+  for (int s0 = lo; s0 < hi; s0 += kU) {
+    A t[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) t[u] = f(min(s0 + u, hi - 1));
+#pragma unroll
+    for (int u = 0; u < kU; u++)
+      if (s0 + u < hi) acc_add(acc, t[u]);
+  }
+  return acc;
+}
+
+template <int K, typename TI, bool C>
+__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, double (&acc)[3]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  TI* r = (TI*)I.r;
-  TI* z = (TI*)I.z;
   TI* xi = (TI*)I.xi;
-  const TI* dinv = (const TI*)I.dinvi;
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
-  const int* __restrict__ rp = I.rp;
-  const int* __restrict__ ci = I.ci;
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  const double GA* M64 = gp(I.M64);
+  const double GA* K64 = gp(I.K64);
+  const double GA* S64 = gp(I.S64);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const int lo = rp[row], hi = rp[row + 1];
     const size_t o = (size_t)row * K + jr;
@@ -168,230 +308,287 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, double (&acc)[3
     switch (I.eq) {
       case EQ_HEAT:
       case EQ_ADVDIFF: {  // b = (M/dt) u                       CPUSolver.cpp:78,96
-        double bs = 0.0;
-        for (int s = lo; s < hi; s++) {
-          const int c = ci[s];
-          const double uc = I.u[(size_t)c * K + jr];
-          bs += I.M64[s] * uc;
-          if (warm) sx += I.S64[(size_t)s * kv + vj] * uc;
-        }
-        b = bs * inv_dt;
-        x0 = warm ? I.u[o] : 0.0;
+        const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
+          const double uc = ld<C>(I.u + (size_t)ci[s] * K + jr);
+          return D2{M64[s] * uc, warm ? S64[(size_t)s * kv + vj] * uc : 0.0};
+        });
+        b = g.a * inv_dt;
+        sx = g.b;
+        x0 = warm ? ld<C>(I.u + o) : 0.0;
       } break;
       case EQ_WAVE: {  // b = (M/dt) v - c^2 K u                    CPUSolver.cpp:114
-        double bs = 0.0, ks = 0.0;
-        for (int s = lo; s < hi; s++) {
-          const int c = ci[s];
-          const double vc = I.v[(size_t)c * K + jr];
-          bs += I.M64[s] * vc;
-          ks += I.K64[s] * I.u[(size_t)c * K + jr];
-          if (warm) sx += I.S64[(size_t)s * kv + vj] * vc;
-        }
-        b = bs * inv_dt - I.p1 * I.p1 * ks;
-        x0 = warm ? I.v[o] : 0.0;
+        const D3 g = row_gather(lo, hi, D3{0.0, 0.0, 0.0}, [&](int s) {
+          const size_t oc = (size_t)ci[s] * K + jr;
+          const double vc = ld<C>(I.v + oc);
+          return D3{M64[s] * vc, K64[s] * ld<C>(I.u + oc), warm ? S64[(size_t)s * kv + vj] * vc : 0.0};
+        });
+        b = g.a * inv_dt - I.p1 * I.p1 * g.b;
+        sx = g.c;
+        x0 = warm ? ld<C>(I.v + o) : 0.0;
       } break;
       case EQ_GS: {  // b_u = (M/dt)u - uvv + F(1-u); b_v = (M/dt)v + uvv - (F+k)v   CPUSolver.cpp:134,137
-        double bs = 0.0;
-        for (int s = lo; s < hi; s++) {
-          const int c = ci[s];
-          const double yc = I.u[(size_t)c * K + jr];
-          bs += I.M64[s] * yc;
-          if (warm) sx += I.S64[(size_t)s * kv + vj] * yc;
-        }
-        const double uu = I.u[(size_t)row * K], vv = I.u[(size_t)row * K + 1];
+        const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
+          const double yc = ld<C>(I.u + (size_t)ci[s] * K + jr);
+          return D2{M64[s] * yc, warm ? S64[(size_t)s * kv + vj] * yc : 0.0};
+        });
+        const double uu = ld<C>(I.u + (size_t)row * K), vv = ld<C>(I.u + (size_t)row * K + 1);
         const double uvv = uu * (vv * vv);
         const double f = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
-        const double w = I.weighting ? I.lumped[row] : 1.0;
-        b = bs * inv_dt + w * f;
-        x0 = warm ? I.u[o] : 0.0;
+        const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
+        b = g.a * inv_dt + w * f;
+        sx = g.b;
+        x0 = warm ? ld<C>(I.u + o) : 0.0;
       } break;
       case EQ_NEWMARK: {  // b = -c^2 K (u + dt v + (1/2 - beta) dt^2 a)
         const double h = (0.5 - I.p2) * I.dt * I.dt;
-        double ks = 0.0;
-        for (int s = lo; s < hi; s++) {
-          const int c = ci[s];
-          const size_t oc = (size_t)c * K + jr;
-          ks += I.K64[s] * (I.u[oc] + I.dt * I.v[oc] + h * I.a[oc]);
-          if (warm) sx += I.S64[(size_t)s * kv + vj] * I.a[oc];
-        }
-        b = -I.p1 * I.p1 * ks;
-        x0 = warm ? I.a[o] : 0.0;
+        const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
+          const size_t oc = (size_t)ci[s] * K + jr;
+          const double ac = ld<C>(I.a + oc);
+          return D2{K64[s] * (ld<C>(I.u + oc) + I.dt * ld<C>(I.v + oc) + h * ac),
+                    warm ? S64[(size_t)s * kv + vj] * ac : 0.0};
+        });
+        b = -I.p1 * I.p1 * g.a;
+        sx = g.b;
+        x0 = warm ? ld<C>(I.a + o) : 0.0;
       } break;
       default: {  // EQ_SOLVE: b supplied, x holds the guess
-        b = I.b[o];
+        b = gp(I.b)[o];
         if (warm)
-          for (int s = lo; s < hi; s++) sx += I.S64[(size_t)s * kv + vj] * I.x[(size_t)ci[s] * K + jr];
-        x0 = warm ? I.x[o] : 0.0;
+          sx = row_gather(lo, hi, 0.0, [&](int s) {
+            return S64[(size_t)s * kv + vj] * ld<C>(I.x + (size_t)ci[s] * K + jr);
+          });
+        x0 = warm ? ld<C>(I.x + o) : 0.0;
       } break;
     }
     const double r64 = b - sx;
-    I.b[o] = b;
-    I.x[o] = x0;
+    gp(I.b)[o] = b;
+    st<C>(I.x + o, x0);
     const TI rr = to_ti<TI>(r64);
-    const TI zz = dinv[(size_t)row * kv + vj] * rr;
-    r[o] = rr;
-    z[o] = zz;
-    xi[o] = (TI)0;
+    st<C>(r + o, rr);
+    st<C>(xi + o, (TI)0);
     acc[0] += b * b;
     acc[1] += r64 * r64;
-    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)zz;
+    // BiCGSTAB starts from rho = <r, r>; CG from <r, D^-1 r>
+    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
   }
 }
 
-// FP64 refinement residual r = b - S64 (x + xi); restarts the inner solve.
-template <int K, typename TI>
-__device__ void phase_res(const Inst& I, int row_lo, int row_hi, double (&acc)[2]) {
+// FP64 refinement residual r = b - S64 (x + xi).
+template <int K, typename TI, bool C>
+__device__ void phase_res(const Inst& I, int row_lo, int row_hi, TI* r, double (&acc)[2]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  TI* r = (TI*)I.r;
-  TI* z = (TI*)I.z;
   const TI* xi = (const TI*)I.xi;
-  const TI* dinv = (const TI*)I.dinvi;
-  const int* __restrict__ rp = I.rp;
-  const int* __restrict__ ci = I.ci;
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  const double GA* S64 = gp(I.S64);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
-    double sx = 0.0;
-    for (int s = rp[row]; s < rp[row + 1]; s++) {
+    const double bo = gp(I.b)[o];
+    const double sx = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
       const size_t oc = (size_t)ci[s] * K + jr;
-      sx += I.S64[(size_t)s * kv + vj] * (I.x[oc] + (double)xi[oc]);
-    }
-    const double r64 = I.b[o] - sx;
+      return S64[(size_t)s * kv + vj] * (ld<C>(I.x + oc) + (double)ld<C>(xi + oc));
+    });
+    const double r64 = bo - sx;
     const TI rr = to_ti<TI>(r64);
-    const TI zz = dinv[(size_t)row * kv + vj] * rr;
-    r[o] = rr;
-    z[o] = zz;
+    st<C>(r + o, rr);
     acc[0] += r64 * r64;
-    acc[1] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)zz;
+    acc[1] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
   }
 }
 
-// CG phase A: p = z + beta p_old, q = S p, <p, q>.  Neighbour p values are
-// recomputed from z and p_old, so p needs no barrier of its own.  On the
-// first iteration of a sweep the pending correction is folded: x += xi.
-template <int K, typename TI>
-__device__ void phase_cg_a(const Inst& I, int row_lo, int row_hi, const TI* pold, TI* pnew, bool first,
-                           const Shared& sh, double (&acc)[1]) {
+// Pipelined CG start: fold x += xi; w = S D^-1 r; <r, D^-1 r>, <D^-1 r, w>.
+template <int K, typename TI, bool C>
+__device__ void phase_pcg_init(const Inst& I, int row_lo, int row_hi, const TI* rc, TI* wc, double (&acc)[2]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  const TI* z = (const TI*)I.z;
-  TI* q = (TI*)I.q;
   TI* xi = (TI*)I.xi;
-  const TI* S = (const TI*)I.Si;
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    const double xo = ld<C>(I.x + o), xio = (double)ld<C>(xi + o);
+    const TI ro = ld<C>(rc + o);
+    const TI wa = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
+      const int c = ci[s];
+      return S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * ld<C>(rc + (size_t)c * K + jr));
+    });
+    st<C>(I.x + o, xo + xio);
+    st<C>(xi + o, (TI)0);
+    st<C>(wc + o, wa);
+    const TI uo = dinv[(size_t)row * kv + vj] * ro;
+    acc[0] += (double)ro * (double)uo;
+    acc[1] += (double)uo * (double)wa;
+  }
+}
+
+// One Chronopoulos-Gear iteration (see file header).
+template <int K, typename TI, bool C>
+__device__ void phase_pcg_iter(const Inst& I, int row_lo, int row_hi, const TI* rc, const TI* wc, const TI* sc,
+                               TI* rn, TI* wn, TI* sn, bool first, const Shared& sh, double (&acc)[3]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  TI* xi = (TI*)I.xi;
+  TI GA* p = gp((TI*)I.p0);  // own rows only
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const TI al = to_ti<TI>(sh.alpha[jr]);
+  const TI be = first ? (TI)0 : to_ti<TI>(sh.beta[jr]);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    const TI ro = ld<C>(rc + o);
+    const TI wo = ld<C>(wc + o);
+    const TI so = first ? (TI)0 : ld<C>(sc + o);
+    const TI po = first ? (TI)0 : p[o];
+    const TI xio = ld<C>(xi + o);
+    const TI wa = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
+      const int c = ci[s];
+      const size_t oc = (size_t)c * K + jr;
+      const TI sv = cg_s<TI>(ld<C>(wc + oc), first ? (TI)0 : ld<C>(sc + oc), be, first);
+      return S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * cg_r<TI>(ld<C>(rc + oc), sv, al));
+    });
+    const TI d = dinv[(size_t)row * kv + vj];
+    const TI uo = d * ro;
+    const TI pn = first ? uo : (uo + be * po);
+    const TI sv = cg_s<TI>(wo, so, be, first);
+    const TI rr = cg_r<TI>(ro, sv, al);
+    p[o] = pn;
+    st<C>(xi + o, xio + al * pn);
+    st<C>(sn + o, sv);
+    st<C>(rn + o, rr);
+    st<C>(wn + o, wa);
+    const TI un = d * rr;
+    acc[0] += (double)rr * (double)un;
+    acc[1] += (double)un * (double)wa;
+    acc[2] += (double)rr * (double)rr;
+  }
+}
+
+// Classic CG phase A: p = z + beta p_old (z = D^-1 r), q = S p, <p, q>.
+// Neighbour p values are recomputed from r and p_old, so p needs no barrier of
+// its own.  On the first iteration of a sweep the pending correction is folded.
+template <int K, typename TI, bool C>
+__device__ void phase_cg_a(const Inst& I, int row_lo, int row_hi, const TI* r, const TI* pold, TI* pnew,
+                           bool first, const Shared& sh, double (&acc)[1]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  TI GA* q = gp((TI*)I.q);  // own rows only
+  TI* xi = (TI*)I.xi;
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const TI bj = first ? (TI)0 : to_ti<TI>(sh.beta[jr]);
-  const int* __restrict__ rp = I.rp;
-  const int* __restrict__ ci = I.ci;
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
     if (first) {
-      I.x[o] += (double)xi[o];
-      xi[o] = (TI)0;
+      st<C>(I.x + o, ld<C>(I.x + o) + (double)ld<C>(xi + o));
+      st<C>(xi + o, (TI)0);
     }
-    TI qa = (TI)0;
-    if (first) {
-      for (int s = rp[row]; s < rp[row + 1]; s++) qa += S[(size_t)s * kv + vj] * z[(size_t)ci[s] * K + jr];
-    } else {
-      for (int s = rp[row]; s < rp[row + 1]; s++) {
-        const size_t oc = (size_t)ci[s] * K + jr;
-        qa += S[(size_t)s * kv + vj] * (z[oc] + bj * pold[oc]);
-      }
-    }
-    const TI pn = first ? z[o] : (z[o] + bj * pold[o]);
-    pnew[o] = pn;
+    const TI qa = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
+      const int c = ci[s];
+      const size_t oc = (size_t)c * K + jr;
+      const TI zc = dinv[(size_t)c * kv + vj] * ld<C>(r + oc);
+      return S[(size_t)s * kv + vj] * (first ? zc : (zc + bj * ld<C>(pold + oc)));
+    });
+    const TI zo = dinv[(size_t)row * kv + vj] * ld<C>(r + o);
+    const TI pn = first ? zo : (zo + bj * ld<C>(pold + o));
+    st<C>(pnew + o, pn);
     q[o] = qa;
     acc[0] += (double)pn * (double)qa;
   }
 }
 
-// CG phase B: xi += alpha p, r -= alpha q, z = D^-1 r, <r,z>, <r,r>.
-template <int K, typename TI>
-__device__ void phase_cg_b(const Inst& I, int row_lo, int row_hi, const TI* p, const Shared& sh,
+// Classic CG phase B: xi += alpha p, r -= alpha q, <r, D^-1 r>, <r, r>.
+template <int K, typename TI, bool C>
+__device__ void phase_cg_b(const Inst& I, int row_lo, int row_hi, TI* r, const TI* p, const Shared& sh,
                            double (&acc)[2]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  TI* r = (TI*)I.r;
-  TI* z = (TI*)I.z;
   TI* xi = (TI*)I.xi;
-  const TI* q = (const TI*)I.q;
-  const TI* dinv = (const TI*)I.dinvi;
+  const TI GA* q = gp((const TI*)I.q);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const TI al = to_ti<TI>(sh.alpha[jr]);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
-    xi[o] += al * p[o];
-    const TI rr = r[o] - al * q[o];
-    const TI zz = dinv[(size_t)row * kv + vj] * rr;
-    r[o] = rr;
-    z[o] = zz;
-    acc[0] += (double)rr * (double)zz;
+    st<C>(xi + o, ld<C>(xi + o) + al * ld<C>(p + o));
+    const TI rr = ld<C>(r + o) - al * q[o];
+    st<C>(r + o, rr);
+    acc[0] += (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
     acc[1] += (double)rr * (double)rr;
   }
 }
 
 // BiCGSTAB (right Jacobi preconditioning) phase P:
 // p = r + beta (p_old - omega v_old); v = S D^-1 p; <rh, v>.
-template <int K, typename TI>
-__device__ void phase_bi_p(const Inst& I, int row_lo, int row_hi, const TI* pold, TI* pnew, const TI* vold,
-                           TI* vnew, bool first, const Shared& sh, double (&acc)[1]) {
+template <int K, typename TI, bool C>
+__device__ void phase_bi_p(const Inst& I, int row_lo, int row_hi, const TI* r, const TI* pold, TI* pnew,
+                           const TI* vold, TI* vnew, bool first, const Shared& sh, double (&acc)[1]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  const TI* r = (const TI*)I.r;
-  TI* rh = (TI*)I.rh;
+  TI GA* rh = gp((TI*)I.rh);  // own rows only
   TI* xi = (TI*)I.xi;
-  const TI* S = (const TI*)I.Si;
-  const TI* dinv = (const TI*)I.dinvi;
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const TI be = first ? (TI)0 : to_ti<TI>(sh.beta[jr]);
   const TI om = first ? (TI)0 : to_ti<TI>(sh.omega[jr]);
-  const int* __restrict__ rp = I.rp;
-  const int* __restrict__ ci = I.ci;
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
+    const TI ro = ld<C>(r + o);
     if (first) {
-      I.x[o] += (double)xi[o];
-      xi[o] = (TI)0;
-      rh[o] = r[o];
+      st<C>(I.x + o, ld<C>(I.x + o) + (double)ld<C>(xi + o));
+      st<C>(xi + o, (TI)0);
+      rh[o] = ro;
     }
-    TI va = (TI)0;
-    for (int s = rp[row]; s < rp[row + 1]; s++) {
+    const TI va = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
       const int c = ci[s];
       const size_t oc = (size_t)c * K + jr;
-      const TI pc = first ? r[oc] : (r[oc] + be * (pold[oc] - om * vold[oc]));
-      va += S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * pc);
-    }
-    const TI pn = first ? r[o] : (r[o] + be * (pold[o] - om * vold[o]));
-    pnew[o] = pn;
-    vnew[o] = va;
-    acc[0] += (double)(first ? r[o] : rh[o]) * (double)va;
+      const TI rcv = ld<C>(r + oc);
+      const TI pc = first ? rcv : (rcv + be * (ld<C>(pold + oc) - om * ld<C>(vold + oc)));
+      return S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * pc);
+    });
+    const TI pn = first ? ro : (ro + be * (ld<C>(pold + o) - om * ld<C>(vold + o)));
+    st<C>(pnew + o, pn);
+    st<C>(vnew + o, va);
+    acc[0] += (double)(first ? ro : rh[o]) * (double)va;
   }
 }
 
 // BiCGSTAB phase S: s = r - alpha v; t = S D^-1 s; <t,s>, <t,t>, <s,s>.
-template <int K, typename TI>
-__device__ void phase_bi_s(const Inst& I, int row_lo, int row_hi, const TI* vnew, const Shared& sh,
+template <int K, typename TI, bool C>
+__device__ void phase_bi_s(const Inst& I, int row_lo, int row_hi, const TI* r, const TI* vnew, const Shared& sh,
                            double (&acc)[3]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  const TI* r = (const TI*)I.r;
-  TI* sv = (TI*)I.s;
-  TI* tv = (TI*)I.t;
-  const TI* S = (const TI*)I.Si;
-  const TI* dinv = (const TI*)I.dinvi;
+  TI GA* sv = gp((TI*)I.s);  // own rows only
+  TI GA* tv = gp((TI*)I.t);  // own rows only
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const TI al = to_ti<TI>(sh.alpha[jr]);
-  const int* __restrict__ rp = I.rp;
-  const int* __restrict__ ci = I.ci;
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
-    TI ta = (TI)0;
-    for (int s = rp[row]; s < rp[row + 1]; s++) {
+    const TI so = ld<C>(r + o) - al * ld<C>(vnew + o);
+    const TI ta = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
       const int c = ci[s];
       const size_t oc = (size_t)c * K + jr;
-      ta += S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * (r[oc] - al * vnew[oc]));
-    }
-    const TI so = r[o] - al * vnew[o];
+      return S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * (ld<C>(r + oc) - al * ld<C>(vnew + oc)));
+    });
     sv[o] = so;
     tv[o] = ta;
     acc[0] += (double)ta * (double)so;
@@ -401,26 +598,25 @@ __device__ void phase_bi_s(const Inst& I, int row_lo, int row_hi, const TI* vnew
 }
 
 // BiCGSTAB phase X: xi += D^-1 (alpha p + omega s); r = s - omega t; <rh,r>, <r,r>.
-template <int K, typename TI>
-__device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, const TI* p, const Shared& sh,
+template <int K, typename TI, bool C>
+__device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, const TI* p, const Shared& sh,
                            double (&acc)[2]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  TI* r = (TI*)I.r;
   TI* xi = (TI*)I.xi;
-  const TI* rh = (const TI*)I.rh;
-  const TI* sv = (const TI*)I.s;
-  const TI* tv = (const TI*)I.t;
-  const TI* dinv = (const TI*)I.dinvi;
+  const TI GA* rh = gp((const TI*)I.rh);
+  const TI GA* sv = gp((const TI*)I.s);
+  const TI GA* tv = gp((const TI*)I.t);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
   const TI al = to_ti<TI>(sh.alpha[jr]);
   const TI om = to_ti<TI>(sh.omega[jr]);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
     const TI d = dinv[(size_t)row * kv + vj];
-    xi[o] += d * (al * p[o] + om * sv[o]);
+    st<C>(xi + o, ld<C>(xi + o) + d * (al * ld<C>(p + o) + om * sv[o]));
     const TI rr = sv[o] - om * tv[o];
-    r[o] = rr;
+    st<C>(r + o, rr);
     acc[0] += (double)rh[o] * (double)rr;
     acc[1] += (double)rr * (double)rr;
   }
@@ -428,7 +624,7 @@ __device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, const TI* p, c
 
 // State update and the instability rule (CPUSolver.cpp:41-48, widened to
 // flag NaN/Inf separately).
-template <int K, typename TI>
+template <int K, typename TI, bool C>
 __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
@@ -436,34 +632,32 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
   int f = 0;
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
-    double xf = I.x[o] + (double)xi[o];
+    double xf = ld<C>(I.x + o) + (double)ld<C>(xi + o);
     if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
-    I.x[o] = xf;
-    xi[o] = (TI)0;
+    st<C>(I.x + o, xf);
+    st<C>(xi + o, (TI)0);
     double c1 = xf, c2 = 0.0;
     switch (I.eq) {
       case EQ_HEAT:
       case EQ_ADVDIFF:
-        I.u[o] = xf;
+      case EQ_GS:
+        st<C>(I.u + o, xf);
         break;
       case EQ_WAVE: {
-        I.v[o] = xf;
-        const double un = I.u[o] + xf * I.dt;  // CPUSolver.cpp:118
-        I.u[o] = un;
+        st<C>(I.v + o, xf);
+        const double un = ld<C>(I.u + o) + xf * I.dt;  // CPUSolver.cpp:118
+        st<C>(I.u + o, un);
         c2 = un;
       } break;
-      case EQ_GS:
-        I.u[o] = xf;
-        break;
       case EQ_NEWMARK: {
         const double dt = I.dt, be = I.p2, ga = I.p3;
-        const double ao = I.a[o], vo = I.v[o];
-        const double ut = I.u[o] + dt * vo + (0.5 - be) * dt * dt * ao;
+        const double ao = ld<C>(I.a + o), vo = ld<C>(I.v + o);
+        const double ut = ld<C>(I.u + o) + dt * vo + (0.5 - be) * dt * dt * ao;
         const double un = ut + be * dt * dt * xf;
         const double vn = vo + dt * ((1.0 - ga) * ao + ga * xf);
-        I.u[o] = un;
-        I.v[o] = vn;
-        I.a[o] = xf;
+        st<C>(I.u + o, un);
+        st<C>(I.v + o, vn);
+        st<C>(I.a + o, xf);
         c1 = un;
         c2 = vn;
       } break;
@@ -477,10 +671,280 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
 }
 
 // ---------------------------------------------------------------------------
+// Krylov drivers (uniform control flow across the team)
+// ---------------------------------------------------------------------------
+
+// Pipelined (Chronopoulos-Gear) PCG on the current sweep; r starts in I.r.
+template <int K, typename TI, bool C>
+__device__ bool krylov_pcg(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo, int row_hi,
+                           int& its) {
+  TI* rb[2] = {(TI*)I.r, (TI*)I.z};
+  TI* wb[2] = {(TI*)I.q, (TI*)I.rh};
+  TI* sb[2] = {(TI*)I.s, (TI*)I.t};
+  int par = 0;
+  {
+    double acc[2] = {0.0, 0.0};
+    phase_pcg_init<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], acc);
+    if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+  }
+  if (threadIdx.x < K) {
+    const int j = threadIdx.x;
+    const double g = sh.red[j], d = sh.red[K + j];
+    sh.gamma[j] = g;
+    sh.alpha[j] = (sh.active[j] && d != 0.0) ? g / d : 0.0;
+    sh.beta[j] = 0.0;
+  }
+  __syncthreads();
+  int it_s = 0;
+  bool first = true;
+  while (true) {
+    {
+      double acc[3] = {0.0, 0.0, 0.0};
+      phase_pcg_iter<K, TI, C>(I, row_lo, row_hi, rb[par], wb[par], sb[par], rb[par ^ 1], wb[par ^ 1], sb[par ^ 1],
+                               first, sh, acc);
+      if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
+    }
+    par ^= 1;
+    first = false;
+    it_s++;
+    its++;
+    if (threadIdx.x < K) {
+      const int j = threadIdx.x;
+      const double g = sh.red[j], d = sh.red[K + j], rr = sh.red[2 * K + j];
+      const double gold = sh.gamma[j], aold = sh.alpha[j];
+      const bool done = !sh.active[j] || rr < sh.ithr[j] || !(rr == rr);
+      double be = 0.0, al = 0.0;
+      if (!done && gold != 0.0 && aold != 0.0) {
+        be = g / gold;
+        const double den = d - be * g / aold;
+        al = (den != 0.0) ? g / den : 0.0;
+      }
+      sh.beta[j] = be;
+      sh.alpha[j] = al;
+      sh.gamma[j] = g;
+      sh.omega[j] = done ? 1.0 : 0.0;  // per-column done flag for the vote below
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int alldone = 1;
+      for (int j = 0; j < K; j++) alldone &= (sh.omega[j] != 0.0);
+      sh.uni = alldone || it_s >= I.max_inner || its >= I.max_iters;
+    }
+    __syncthreads();
+    if (sh.uni) break;
+  }
+  return true;
+}
+
+template <int K, typename TI, bool C>
+__device__ bool krylov_cg_classic(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo,
+                                  int row_hi, int& its) {
+  TI* r = (TI*)I.r;
+  TI* pa = (TI*)I.p0;
+  TI* pb = (TI*)I.p1b;
+  int it_s = 0;
+  bool first = true;
+  while (true) {
+    {
+      double acc[1] = {0.0};
+      phase_cg_a<K, TI, C>(I, row_lo, row_hi, r, pa, pb, first, sh, acc);
+      if (!team_reduce<K, 1, C>(acc, T, rank, e, sh)) return false;
+    }
+    if (threadIdx.x < K) {
+      const double pq = sh.red[threadIdx.x];
+      sh.alpha[threadIdx.x] = (sh.active[threadIdx.x] && pq != 0.0) ? sh.gamma[threadIdx.x] / pq : 0.0;
+    }
+    __syncthreads();
+    {
+      double acc[2] = {0.0, 0.0};
+      phase_cg_b<K, TI, C>(I, row_lo, row_hi, r, pb, sh, acc);
+      if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+    }
+    it_s++;
+    its++;
+    if (threadIdx.x < K) {
+      const double rz = sh.red[threadIdx.x];
+      const double g = sh.gamma[threadIdx.x];
+      sh.beta[threadIdx.x] = (g != 0.0) ? rz / g : 0.0;
+      sh.gamma[threadIdx.x] = rz;
+    }
+    if (threadIdx.x == 0) {
+      int alldone = 1;
+      for (int j = 0; j < K; j++)
+        alldone &= (!sh.active[j]) || (sh.red[K + j] < sh.ithr[j]) || !(sh.red[K + j] == sh.red[K + j]);
+      sh.uni = alldone || it_s >= I.max_inner || its >= I.max_iters;
+    }
+    __syncthreads();
+    if (sh.uni) break;
+    first = false;
+    TI* tmp = pa;
+    pa = pb;
+    pb = tmp;
+  }
+  return true;
+}
+
+template <int K, typename TI, bool C>
+__device__ bool krylov_bicgstab(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo,
+                                int row_hi, int& its) {
+  TI* r = (TI*)I.r;
+  TI* pa = (TI*)I.p0;
+  TI* pb = (TI*)I.p1b;
+  TI* va = (TI*)I.v0;
+  TI* vb = (TI*)I.v1;
+  int it_s = 0;
+  bool first = true;
+  while (true) {
+    {
+      double acc[1] = {0.0};
+      phase_bi_p<K, TI, C>(I, row_lo, row_hi, r, pa, pb, va, vb, first, sh, acc);
+      if (!team_reduce<K, 1, C>(acc, T, rank, e, sh)) return false;
+    }
+    if (threadIdx.x < K) {
+      const double rv = sh.red[threadIdx.x];
+      sh.alpha[threadIdx.x] = (sh.active[threadIdx.x] && rv != 0.0) ? sh.gamma[threadIdx.x] / rv : 0.0;
+    }
+    __syncthreads();
+    {
+      double acc[3] = {0.0, 0.0, 0.0};
+      phase_bi_s<K, TI, C>(I, row_lo, row_hi, r, vb, sh, acc);
+      if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
+    }
+    if (threadIdx.x < K) {
+      const double ts = sh.red[threadIdx.x], tt = sh.red[K + threadIdx.x], ss = sh.red[2 * K + threadIdx.x];
+      const bool act = sh.active[threadIdx.x];
+      sh.omega[threadIdx.x] = (act && tt > 0.0 && !(ss < sh.ithr[threadIdx.x])) ? ts / tt : 0.0;
+    }
+    __syncthreads();
+    {
+      double acc[2] = {0.0, 0.0};
+      phase_bi_x<K, TI, C>(I, row_lo, row_hi, r, pb, sh, acc);
+      if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+    }
+    it_s++;
+    its++;
+    if (threadIdx.x == 0) {
+      int alldone = 1, brk = 0;
+      for (int j = 0; j < K; j++) {
+        const double rho_new = sh.red[j], rr = sh.red[K + j];
+        const bool act = sh.active[j];
+        const bool done = !act || rr < sh.ithr[j] || !(rr == rr);
+        alldone &= done;
+        if (!done) {
+          const double om = sh.omega[j], rho = sh.gamma[j];
+          if (om == 0.0 || rho == 0.0 || rho_new == 0.0) brk = 1;  // breakdown: restart sweep
+          else sh.beta[j] = (rho_new / rho) * (sh.alpha[j] / om);
+        }
+        sh.gamma[j] = rho_new;
+      }
+      sh.uni = alldone || brk || it_s >= I.max_inner || its >= I.max_iters;
+    }
+    __syncthreads();
+    if (sh.uni) break;
+    first = false;
+    TI* tmp = pa;
+    pa = pb;
+    pb = tmp;
+    tmp = va;
+    va = vb;
+    vb = tmp;
+  }
+  return true;
+}
+
+// All steps of one instance by one team.  KR: 0 pipelined CG, 1 BiCGSTAB,
+// 2 classic CG (one Krylov method per kernel keeps register pressure down).
+// Returns false when the launch aborts.
+template <int K, typename TI, int KR, bool C>
+__device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int nsteps) {
+  const int n = I.n;
+  const int rows_per_wg = (n + T.size - 1) / T.size;
+  const int row_lo = min(n, rank * rows_per_wg);
+  const int row_hi = min(n, row_lo + rows_per_wg);
+  long long its_total = 0, sweeps_total = 0;
+  int status = 0;
+  TI* r0 = (TI*)I.r;
+  for (int step = 0; step < nsteps; step++) {
+    // ---- R: right-hand side and initial residual
+    {
+      double acc[3] = {0.0, 0.0, 0.0};
+      phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, acc);
+      if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
+    }
+    if (threadIdx.x < K) {
+      const double bb = sh.red[threadIdx.x], rr = sh.red[K + threadIdx.x];
+      sh.bb[threadIdx.x] = bb;
+      sh.thr[threadIdx.x] = fmax(I.tol2 * bb, DBL_MIN);
+      sh.active[threadIdx.x] = (bb != 0.0) && (rr >= sh.thr[threadIdx.x]);
+      sh.gamma[threadIdx.x] = sh.red[2 * K + threadIdx.x];
+      sh.ithr[threadIdx.x] = rr;  // residual^2 at sweep start
+    }
+    __syncthreads();
+    int sweeps = 0, its = 0;
+    while (true) {
+      if (threadIdx.x == 0) {
+        int any = 0;
+        for (int j = 0; j < K; j++) any |= sh.active[j];
+        sh.uni = any && sweeps < I.max_sweeps && its < I.max_iters;
+      }
+      __syncthreads();
+      if (!sh.uni) break;
+      if (threadIdx.x < K) {
+        // FP32 sweeps reduce the residual by inner_tol; an FP64 inner
+        // solve goes straight to the final tolerance.
+        const double rs = sh.ithr[threadIdx.x];
+        sh.ithr[threadIdx.x] = (sizeof(TI) == 4) ? fmax(I.inner_tol2 * rs, sh.thr[threadIdx.x])
+                                                 : sh.thr[threadIdx.x];
+      }
+      __syncthreads();
+      bool ok;
+      if constexpr (KR == 1) ok = krylov_bicgstab<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
+      else if constexpr (KR == 2) ok = krylov_cg_classic<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
+      else ok = krylov_pcg<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
+      if (!ok) return false;
+      // ---- Res: FP64 residual of x + xi
+      {
+        double acc[2] = {0.0, 0.0};
+        phase_res<K, TI, C>(I, row_lo, row_hi, r0, acc);
+        if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+      }
+      sweeps++;
+      if (threadIdx.x < K) {
+        const double rr = sh.red[threadIdx.x];
+        sh.active[threadIdx.x] = (sh.bb[threadIdx.x] != 0.0) && (rr >= sh.thr[threadIdx.x]) && (rr == rr);
+        sh.gamma[threadIdx.x] = sh.red[K + threadIdx.x];
+        sh.ithr[threadIdx.x] = rr;
+      }
+      __syncthreads();
+    }
+    // ---- U: state update
+    int notconv = 0;
+    for (int j = 0; j < K; j++) notconv |= sh.active[j];
+    if (notconv) status |= ST_NOT_CONVERGED;
+    phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status);
+    if (rank == 0 && threadIdx.x < K) {
+      const double bb = sh.bb[threadIdx.x];
+      I.last_res[threadIdx.x] = bb > 0.0 ? sqrt(sh.ithr[threadIdx.x] / bb) : 0.0;
+    }
+    its_total += its;
+    sweeps_total += sweeps;
+    // the next step (or the next instance of this team) gathers the new state
+    if (!team_barrier(T, rank, e, sh)) return false;
+  }
+  if (rank == 0 && threadIdx.x == 0) {
+    I.counters[0] += its_total;
+    I.counters[1] += sweeps_total;
+    I.counters[2] += nsteps;
+    if (status) atomicOr(I.status, status);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // The kernel
 // ---------------------------------------------------------------------------
 
-template <int K, typename TI>
+template <int K, typename TI, int KR, bool C>
 __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* wg_team, const Inst* insts,
                                                 const int* team_inst, int nsteps) {
   __shared__ Shared sh;
@@ -489,205 +953,40 @@ __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* w
   const Team T = teams[tid_team];
   const int l = (int)blockIdx.x - T.first_wg;
   const int rank = xcd_rank(l, T.size);
-  double* my_partial = T.partial + (size_t)rank * kMaxRed;
-
+  unsigned e = 0;  // barrier epoch, identical across the team
   for (int qi = T.inst_begin; qi < T.inst_end; qi++) {
     const Inst& I = insts[team_inst[qi]];
-    const int n = I.n;
-    if (n <= 0) continue;
-    const int rows_per_wg = (n + T.size - 1) / T.size;
-    const int row_lo = min(n, rank * rows_per_wg);
-    const int row_hi = min(n, row_lo + rows_per_wg);
-    const int jr = threadIdx.x % K;
-    long long its_total = 0, sweeps_total = 0;
-    int status = 0;
+    if (I.n <= 0) continue;
+    if (!run_instance<K, TI, KR, C>(I, T, rank, e, sh, nsteps)) return;
+  }
+}
 
-    for (int step = 0; step < nsteps; step++) {
-      // ---- R: right-hand side and initial residual
-      {
-        double acc[3] = {0.0, 0.0, 0.0};
-        phase_rhs<K, TI>(I, row_lo, row_hi, acc);
-        block_partials<K, 3>(acc, sh, my_partial);
-        if (!team_sync(T, 3 * K, sh)) return;
-      }
-      if (threadIdx.x < K) {
-        const double bb = sh.red[threadIdx.x], rr = sh.red[K + threadIdx.x];
-        sh.bb[threadIdx.x] = bb;
-        sh.thr[threadIdx.x] = fmax(I.tol2 * bb, DBL_MIN);
-        sh.active[threadIdx.x] = (bb != 0.0) && (rr >= sh.thr[threadIdx.x]);
-        sh.gamma[threadIdx.x] = sh.red[2 * K + threadIdx.x];
-        sh.ithr[threadIdx.x] = rr;  // residual^2 at sweep start
-      }
-      __syncthreads();
-      int sweeps = 0, its = 0;
-      const int max_iters = I.max_iters;
-      while (true) {
-        if (threadIdx.x == 0) {
-          int any = 0;
-          for (int j = 0; j < K; j++) any |= sh.active[j];
-          sh.uni = any && sweeps < I.max_sweeps && its < max_iters;
-        }
-        __syncthreads();
-        if (!sh.uni) break;
-        if (threadIdx.x < K) {
-          // inner threshold: FP32 sweeps reduce the residual by inner_tol;
-          // an FP64 inner solve goes straight to the final tolerance.
-          const double r0 = sh.ithr[threadIdx.x];
-          sh.ithr[threadIdx.x] = (sizeof(TI) == 4) ? fmax(I.inner_tol2 * r0, sh.thr[threadIdx.x])
-                                                   : sh.thr[threadIdx.x];
-        }
-        __syncthreads();
-        int it_s = 0;
-        TI* pa = (TI*)I.p0;
-        TI* pb = (TI*)I.p1b;
-        if (I.krylov == 0) {
-          // ---------------- preconditioned CG
-          bool first = true;
-          while (true) {
-            {
-              double acc[1] = {0.0};
-              phase_cg_a<K, TI>(I, row_lo, row_hi, pa, pb, first, sh, acc);
-              block_partials<K, 1>(acc, sh, my_partial);
-              if (!team_sync(T, K, sh)) return;
-            }
-            if (threadIdx.x < K) {
-              const double pq = sh.red[threadIdx.x];
-              sh.alpha[threadIdx.x] =
-                  (sh.active[threadIdx.x] && pq != 0.0) ? sh.gamma[threadIdx.x] / pq : 0.0;
-            }
-            __syncthreads();
-            {
-              double acc[2] = {0.0, 0.0};
-              phase_cg_b<K, TI>(I, row_lo, row_hi, pb, sh, acc);
-              block_partials<K, 2>(acc, sh, my_partial);
-              if (!team_sync(T, 2 * K, sh)) return;
-            }
-            it_s++;
-            its++;
-            if (threadIdx.x < K) {
-              const double rz = sh.red[threadIdx.x];
-              const double g = sh.gamma[threadIdx.x];
-              sh.beta[threadIdx.x] = (g != 0.0) ? rz / g : 0.0;
-              sh.gamma[threadIdx.x] = rz;
-            }
-            if (threadIdx.x == 0) {
-              int alldone = 1;
-              for (int j = 0; j < K; j++)
-                alldone &= (!sh.active[j]) || (sh.red[K + j] < sh.ithr[j]) || !(sh.red[K + j] == sh.red[K + j]);
-              sh.uni = alldone || it_s >= I.max_inner || its >= max_iters;
-            }
-            __syncthreads();
-            if (sh.uni) break;
-            first = false;
-            TI* tmp = pa;
-            pa = pb;
-            pb = tmp;
-          }
-        } else {
-          // ---------------- BiCGSTAB
-          TI* va = (TI*)I.v0;
-          TI* vb = (TI*)I.v1;
-          bool first = true;
-          while (true) {
-            {
-              double acc[1] = {0.0};
-              phase_bi_p<K, TI>(I, row_lo, row_hi, pa, pb, va, vb, first, sh, acc);
-              block_partials<K, 1>(acc, sh, my_partial);
-              if (!team_sync(T, K, sh)) return;
-            }
-            if (threadIdx.x < K) {
-              const double rv = sh.red[threadIdx.x];
-              sh.alpha[threadIdx.x] =
-                  (sh.active[threadIdx.x] && rv != 0.0) ? sh.gamma[threadIdx.x] / rv : 0.0;
-            }
-            __syncthreads();
-            {
-              double acc[3] = {0.0, 0.0, 0.0};
-              phase_bi_s<K, TI>(I, row_lo, row_hi, vb, sh, acc);
-              block_partials<K, 3>(acc, sh, my_partial);
-              if (!team_sync(T, 3 * K, sh)) return;
-            }
-            if (threadIdx.x < K) {
-              const double ts = sh.red[threadIdx.x], tt = sh.red[K + threadIdx.x], ss = sh.red[2 * K + threadIdx.x];
-              const bool act = sh.active[threadIdx.x];
-              sh.omega[threadIdx.x] = (act && tt > 0.0 && !(ss < sh.ithr[threadIdx.x])) ? ts / tt : 0.0;
-            }
-            __syncthreads();
-            {
-              double acc[2] = {0.0, 0.0};
-              phase_bi_x<K, TI>(I, row_lo, row_hi, pb, sh, acc);
-              block_partials<K, 2>(acc, sh, my_partial);
-              if (!team_sync(T, 2 * K, sh)) return;
-            }
-            it_s++;
-            its++;
-            if (threadIdx.x == 0) {
-              int alldone = 1, brk = 0;
-              for (int j = 0; j < K; j++) {
-                const double rho_new = sh.red[j], rr = sh.red[K + j];
-                const bool act = sh.active[j];
-                const bool done = !act || rr < sh.ithr[j] || !(rr == rr);
-                alldone &= done;
-                if (!done) {
-                  const double om = sh.omega[j], rho = sh.gamma[j];
-                  if (om == 0.0 || rho == 0.0 || rho_new == 0.0) brk = 1;  // breakdown: restart sweep
-                  else sh.beta[j] = (rho_new / rho) * (sh.alpha[j] / om);
-                }
-                sh.gamma[j] = rho_new;
-              }
-              sh.uni = alldone || brk || it_s >= I.max_inner || its >= max_iters;
-            }
-            __syncthreads();
-            if (sh.uni) break;
-            first = false;
-            TI* tmp = pa;
-            pa = pb;
-            pb = tmp;
-            tmp = va;
-            va = vb;
-            vb = tmp;
-          }
-        }
-        // ---- Res: FP64 residual of x + xi
-        {
-          double acc[2] = {0.0, 0.0};
-          phase_res<K, TI>(I, row_lo, row_hi, acc);
-          block_partials<K, 2>(acc, sh, my_partial);
-          if (!team_sync(T, 2 * K, sh)) return;
-        }
-        sweeps++;
-        if (threadIdx.x < K) {
-          const double rr = sh.red[threadIdx.x];
-          sh.active[threadIdx.x] = (sh.bb[threadIdx.x] != 0.0) && (rr >= sh.thr[threadIdx.x]) && (rr == rr);
-          if (!(rr == rr)) sh.active[threadIdx.x] = 0;
-          sh.gamma[threadIdx.x] = sh.red[K + threadIdx.x];
-          sh.ithr[threadIdx.x] = rr;
-        }
-        __syncthreads();
-      }
-      // ---- U: state update
-      {
-        int notconv = 0;
-        for (int j = 0; j < K; j++) notconv |= sh.active[j];
-        if (notconv) status |= ST_NOT_CONVERGED;
-        phase_update<K, TI>(I, row_lo, row_hi, sh, I.status);
-        if (rank == 0 && threadIdx.x < K) {
-          const double bb = sh.bb[threadIdx.x];
-          I.last_res[threadIdx.x] = bb > 0.0 ? sqrt(sh.ithr[threadIdx.x] / bb) : 0.0;
-        }
-        its_total += its;
-        sweeps_total += sweeps;
-        // the next step (or the next instance of this team) gathers the new state
-        if (!team_sync(T, 0, sh)) return;
-      }
-      (void)jr;
-    }
-    if (rank == 0 && threadIdx.x == 0) {
-      I.counters[0] += its_total;
-      I.counters[1] += sweeps_total;
-      I.counters[2] += nsteps;
-      if (status) atomicOr(I.status, status);
-    }
+template <int K, typename TI>
+const void* pick_kr(int kr, int coh) {
+  switch (kr * 2 + (coh ? 1 : 0)) {
+    case 0: return (const void*)k_step<K, TI, 0, false>;
+    case 1: return (const void*)k_step<K, TI, 0, true>;
+    case 2: return (const void*)k_step<K, TI, 1, false>;
+    case 3: return (const void*)k_step<K, TI, 1, true>;
+    case 4: return (const void*)k_step<K, TI, 2, false>;
+    case 5: return (const void*)k_step<K, TI, 2, true>;
+    default: return nullptr;
+  }
+}
+
+template <int K>
+const void* pick_ti(int fp32, int kr, int coh) {
+  return fp32 ? pick_kr<K, float>(kr, coh) : pick_kr<K, double>(kr, coh);
+}
+
+// Kernel instantiation for (columns, inner precision, Krylov method, coherent mode).
+const void* pick_kernel(int k, int fp32, int kr, int coh) {
+  switch (k) {
+    case 1: return pick_ti<1>(fp32, kr, coh);
+    case 2: return pick_ti<2>(fp32, kr, coh);
+    case 4: return pick_ti<4>(fp32, kr, coh);
+    case 16: return pick_ti<16>(fp32, kr, coh);
+    default: return nullptr;
   }
 }
 
@@ -696,43 +995,19 @@ __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* w
 // Launch one persistent cooperative grid for a prepared batch.
 int launch_step(const StepLaunch& L, int nsteps, hipStream_t st) {
   void* args[] = {(void*)&L.teams, (void*)&L.wg_team, (void*)&L.insts, (void*)&L.team_inst, (void*)&nsteps};
-  const void* fn = nullptr;
-#define PICK(KK)                                                                        \
-  case KK:                                                                              \
-    fn = L.inner_fp32 ? (const void*)k_step<KK, float> : (const void*)k_step<KK, double>; \
-    break;
-  switch (L.k) {
-    PICK(1)
-    PICK(2)
-    PICK(4)
-    PICK(8)
-    PICK(16)
-    default:
-      set_error("unsupported number of right-hand sides");
-      return FEM_ERR_INVALID;
+  const void* fn = pick_kernel(L.k, L.inner_fp32, L.kr, L.coh);
+  if (!fn) {
+    set_error("unsupported kernel configuration (number of right-hand sides must be 1, 2, 4 or 16)");
+    return FEM_ERR_INVALID;
   }
-#undef PICK
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(kBlock), args, 0, st);
   if (e != hipSuccess) return hip_fail(e, "hipLaunchCooperativeKernel(k_step)");
   return 0;
 }
 
-int step_max_coresident(int k, int inner_fp32, int* blocks) {
-  const void* fn = nullptr;
-#define PICK(KK)                                                                        \
-  case KK:                                                                              \
-    fn = inner_fp32 ? (const void*)k_step<KK, float> : (const void*)k_step<KK, double>; \
-    break;
-  switch (k) {
-    PICK(1)
-    PICK(2)
-    PICK(4)
-    PICK(8)
-    PICK(16)
-    default:
-      return FEM_ERR_INVALID;
-  }
-#undef PICK
+int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks) {
+  const void* fn = pick_kernel(k, inner_fp32, kr, coh);
+  if (!fn) return FEM_ERR_INVALID;
   int per_cu = 0, dev = 0, cus = 0;
   FEM_HIP(hipGetDevice(&dev));
   FEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

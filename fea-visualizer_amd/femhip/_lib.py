@@ -46,6 +46,7 @@ FEM_PREC_FP32_IR = 1
 FEM_KRYLOV_AUTO = 0
 FEM_KRYLOV_CG = 1
 FEM_KRYLOV_BICGSTAB = 2
+FEM_KRYLOV_CG_CLASSIC = 3
 FEM_REACTION_NODAL = 0
 FEM_REACTION_LUMPED = 1
 

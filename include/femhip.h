@@ -61,7 +61,9 @@ enum { FEM_MATRIX_K = 0, FEM_MATRIX_M = 1, FEM_MATRIX_A = 2 };
 /* Solver engines and arithmetic. */
 enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1 };
 enum { FEM_PREC_FP64 = 0, FEM_PREC_FP32_IR = 1 };
-enum { FEM_KRYLOV_AUTO = 0, FEM_KRYLOV_CG = 1, FEM_KRYLOV_BICGSTAB = 2 };
+/* FEM_KRYLOV_CG is single-reduction (Chronopoulos-Gear) PCG;
+ * FEM_KRYLOV_CG_CLASSIC the two-reduction form of Eigen's conjugate_gradient. */
+enum { FEM_KRYLOV_AUTO = 0, FEM_KRYLOV_CG = 1, FEM_KRYLOV_BICGSTAB = 2, FEM_KRYLOV_CG_CLASSIC = 3 };
 /* Gray-Scott reaction weighting: the reference adds the nodal reaction
  * without mass weighting (CPUSolver.cpp:134,137); LUMPED multiplies it by the
  * row sums of M (a documented deviation that is stable at large time steps). */
@@ -103,7 +105,7 @@ typedef struct fem_solver_opts {
   int32_t engine;        /* FEM_ENGINE_PERSISTENT (default) or FEM_ENGINE_PLAIN (FP64 baseline) */
   int32_t precision;     /* FEM_PREC_FP32_IR (default) or FEM_PREC_FP64 */
   int32_t krylov;        /* FEM_KRYLOV_AUTO: CG, BiCGSTAB for advection-diffusion */
-  int32_t nrhs;          /* right-hand sides per instance, 1..16 (Gray-Scott: 1) */
+  int32_t nrhs;          /* right-hand sides per instance: 1, 2, 4 or 16 (Gray-Scott: 1) */
   int32_t warm_start;    /* 1: start from the previous state; 0: x0 = 0 as Eigen's solve() */
   int32_t max_iters;     /* Krylov iterations per solve; 0 = 2N (Eigen's default) */
   int32_t max_inner;     /* FP32 iterations per refinement sweep; 0 = 200 */
