@@ -311,6 +311,19 @@ int fem_solver_sync(fem_solver* s) {
   return FEM_OK;
 }
 
+int fem_solver_autotune(fem_solver* s, int32_t probe_steps, int32_t* chosen, double* best_ms) {
+  FEM_GUARD({
+    if (!s) {
+      set_error("null solver");
+      return FEM_ERR_INVALID;
+    }
+    int c = 0;
+    const int rc = batch_autotune(s->batch, probe_steps, &c, best_ms);
+    if (chosen) *chosen = c;
+    return rc;
+  })
+}
+
 // ------------------------------------------------------------------ batch
 
 int fem_batch_create(fem_ctx* const* ctxs, int32_t count, const fem_solver_opts* opts, fem_batch** out) {
@@ -374,6 +387,19 @@ int fem_batch_sync(fem_batch* b) {
   if (!b) return FEM_ERR_INVALID;
   FEM_HIP(hipStreamSynchronize(b->stream));
   return FEM_OK;
+}
+
+int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, double* best_ms) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    int c = 0;
+    const int rc = batch_autotune(b, probe_steps, &c, best_ms);
+    if (chosen) *chosen = c;
+    return rc;
+  })
 }
 
 }  // extern "C"

@@ -47,13 +47,20 @@ __global__ void k_form(int n, const int* rp, const int* diag, const double* M, c
   }
 }
 
-// Barrier epochs restart at 0 every launch.  (A fill kernel rather than a
-// memset node, so the launch sequence can be captured in a graph.)
-__global__ void k_reset(Slot* slots, int nslots, TeamGen* gens, int nteams, int* abort_flag, Inst* insts,
+// Barrier epochs restart at 1 every launch.  Barrier e uses buffer e & 3 and
+// tag (e >> 2) & 1 in the lowest mantissa bit, so before a launch buffer 0
+// must carry tag 0 and buffers 1-3 tag 1 (none of them looks "arrived").
+// (A fill kernel rather than a memset node, so the launch sequence can be
+// captured in a graph.)
+__global__ void k_reset(Slot* slots, int nslots, double* scal, int nteams, int* abort_flag, Inst* insts,
                         int ninst) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nslots) slots[i].epoch = 0u;
-  if (i < nteams) gens[i].gen = 0u;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double tag1 = __longlong_as_double(1ll);  // smallest denormal: value 0, tag 1
+  const int per = kSlotBufs * kMaxRed;
+  if (i < nslots + nteams) {
+    double* v = (i < nslots) ? &slots[i].vals[0][0] : scal + (size_t)(i - nslots) * per;
+    for (int q = 0; q < per; q++) v[q] = (q < kMaxRed) ? 0.0 : tag1;
+  }
   if (i == 0) *abort_flag = 0;
   if (i < ninst) {
     insts[i].counters[0] = 0;
@@ -226,7 +233,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
 }
 
 void free_plan(fem_batch* b) {
-  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_slots, b->d_gens, b->d_scal, b->d_abort};
+  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_slots, b->d_scal, b->d_abort};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b->d_insts = nullptr;
@@ -234,7 +241,6 @@ void free_plan(fem_batch* b) {
   b->d_wg_team = nullptr;
   b->d_team_inst = nullptr;
   b->d_slots = nullptr;
-  b->d_gens = nullptr;
   b->d_scal = nullptr;
   b->d_abort = nullptr;
   b->plan_ready = false;
@@ -288,7 +294,8 @@ int build_plan(fem_batch* b) {
     teams.push_back({0});
     sizes.push_back(g);
   } else {
-    const double share = W / maxb;
+    const int budget = (b->grid_budget > 0) ? std::min(b->grid_budget, maxb) : maxb;
+    const double share = W / budget;
     std::vector<int> small;
     int used = 0;
     for (int i = 0; i < ninst; i++) {
@@ -301,15 +308,15 @@ int build_plan(fem_batch* b) {
         small.push_back(i);
       }
     }
-    if (used > maxb) {  // rounding guard
-      double f = (double)maxb / (used + (double)small.size());
+    if (used > budget) {  // rounding guard
+      double f = (double)budget / (used + (double)small.size());
       used = 0;
       for (auto& g : sizes) {
         g = std::max(1, (int)(g * f));
         used += g;
       }
     }
-    int bins = std::max(1, std::min((int)small.size(), maxb - used));
+    int bins = std::max(1, std::min((int)small.size(), std::max(1, budget - used)));
     std::sort(small.begin(), small.end(), [&](int x, int y) { return work[x] > work[y]; });
     std::vector<double> load(bins, 0.0);
     std::vector<std::vector<int>> lists(bins);
@@ -347,15 +354,13 @@ int build_plan(fem_batch* b) {
   b->d_wg_team = dalloc<int>(grid, &rc);
   b->d_team_inst = dalloc<int>(team_inst.size(), &rc);
   b->d_slots = dalloc<Slot>(grid, &rc);
-  b->d_gens = dalloc<TeamGen>(nteams, &rc);
-  b->d_scal = dalloc<double>((size_t)nteams * 2 * kMaxRed, &rc);
+  b->d_scal = dalloc<double>((size_t)nteams * kSlotBufs * kMaxRed, &rc);
   b->d_abort = dalloc<int>(1, &rc);
   if (rc) return rc;
   int poff = 0;
   for (int t = 0; t < nteams; t++) {
     ht[t].slots = b->d_slots + poff;
-    ht[t].gen = b->d_gens + t;
-    ht[t].scal = b->d_scal + (size_t)t * 2 * kMaxRed;
+    ht[t].scal = b->d_scal + (size_t)t * kSlotBufs * kMaxRed;
     ht[t].abort_flag = b->d_abort;
     // every workgroup reading every partial costs size^2 * nvals loads per
     // barrier; beyond ~16 KB per workgroup rank 0 reduces and publishes
@@ -364,8 +369,6 @@ int build_plan(fem_batch* b) {
   }
   std::vector<Inst> hi(ninst);
   for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
-  FEM_HIP(hipMemsetAsync(b->d_slots, 0, sizeof(Slot) * grid, b->stream));
-  FEM_HIP(hipMemsetAsync(b->d_gens, 0, sizeof(TeamGen) * nteams, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_teams, ht.data(), sizeof(Team) * nteams, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_wg_team, wg_team.data(), sizeof(int) * grid, hipMemcpyHostToDevice, b->stream));
@@ -669,8 +672,8 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   int rc = batch_prepare(b, true);
   if (rc) return rc;
   const int ninst = (int)b->insts.size();
-  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(std::max(b->nteams, ninst), b->grid), kBlock)), dim3(kBlock), 0,
-                     b->stream, b->d_slots, b->grid, b->d_gens, b->nteams, b->d_abort, b->d_insts, ninst);
+  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(b->nteams + b->grid, ninst), kBlock)), dim3(kBlock), 0,
+                     b->stream, b->d_slots, b->grid, b->d_scal, b->nteams, b->d_abort, b->d_insts, ninst);
   FEM_HIP(hipGetLastError());
   FEM_HIP(hipEventRecord(b->ev0, b->stream));
   if (nsteps > 0) {
@@ -688,6 +691,80 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
+  return rc;
+}
+
+// Autotuner: run `probe` steps under a few launch plans from the same state
+// (snapshotted and restored around every probe) and keep the fastest by
+// kernel time.  One instance: the team size around the one-row-per-lane
+// default; several: the number of workgroups the plan spreads the batch over.
+int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
+  if (b->opts.engine != FEM_ENGINE_PERSISTENT || probe <= 0) return 0;
+  int rc = batch_prepare(b, true);
+  if (rc) return rc;
+  int maxb = 0;
+  rc = step_max_coresident(b->launch.k, b->launch.inner_fp32, b->launch.kr, 1, &maxb);
+  if (rc) return rc;
+  const bool single = b->insts.size() == 1;
+  std::vector<int> cands;
+  if (single) {
+    const int g0 = b->team_sizes[0];
+    for (double f : {0.25, 0.5, 0.75, 1.0, 1.5, 2.0}) {
+      const int g = std::max(1, std::min(maxb, (int)std::lround(g0 * f)));
+      if (std::find(cands.begin(), cands.end(), g) == cands.end()) cands.push_back(g);
+    }
+  } else {
+    for (double f : {0.25, 0.5, 0.75, 1.0}) cands.push_back(std::max(1, (int)(maxb * f)));
+  }
+  // snapshot the state
+  std::vector<double*> snap;
+  for (auto& h : b->insts) {
+    const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
+    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x}) {
+      double* d = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (!d) break;
+      FEM_HIP(hipMemcpyAsync(d, src, vb, hipMemcpyDeviceToDevice, b->stream));
+      snap.push_back(d);
+    }
+  }
+  auto restore = [&]() -> int {
+    size_t q = 0;
+    for (auto& h : b->insts) {
+      const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
+      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x}) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
+    }
+    return 0;
+  };
+  const int saved_team = b->opts.team_size, saved_budget = b->grid_budget;
+  double best = 1e300;
+  int best_c = single ? saved_team : saved_budget;
+  if (!rc) {
+    for (int c : cands) {
+      if (single) b->opts.team_size = c;
+      else b->grid_budget = c;
+      b->plan_ready = false;
+      if ((rc = restore())) break;
+      fem_step_stats st{};
+      const int r = batch_advance(b, probe, &st);
+      if (r < 0) {
+        rc = r;
+        break;
+      }
+      if (st.ms < best) {
+        best = st.ms;
+        best_c = c;
+      }
+    }
+  }
+  const int rr = restore();
+  if (!rc) rc = rr;
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  for (double* d : snap) (void)hipFree(d);
+  if (single) b->opts.team_size = rc ? saved_team : best_c;
+  else b->grid_budget = rc ? saved_budget : best_c;
+  b->plan_ready = false;
+  if (chosen) *chosen = best_c;
+  if (best_ms) *best_ms = best;
   return rc;
 }
 

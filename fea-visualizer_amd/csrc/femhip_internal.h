@@ -37,21 +37,16 @@ enum : int {
   ST_BARRIER_TIMEOUT = 8,
 };
 
-// Per-workgroup barrier slot: the partial sums of the last two barriers
-// (double-buffered by epoch parity) and the epoch this workgroup reached.
-// Arrival is a plain release store to the workgroup's own slot, so arrivals
-// never contend on one address.
+// Per-workgroup barrier slot: the partial sums of the last four barriers.
+// Barrier e writes buffer e & 3, and the lowest mantissa bit of every value
+// carries the tag (e >> 2) & 1, so a reader recognises this barrier's values
+// without a separate flag.  Arrivals go to the workgroup's own slot and never
+// contend on one address.
+constexpr int kSlotBufs = 4;
 struct alignas(128) Slot {
-  double vals[2][kMaxRed];
-  unsigned int epoch;
-  unsigned int pad[31];
+  double vals[kSlotBufs][kMaxRed];
 };
 
-// Release word of a team (master-reduction mode) on its own cache line.
-struct alignas(128) TeamGen {
-  unsigned int gen;
-  unsigned int pad[31];
-};
 
 // Device-side description of one instance (a mesh, its parameters and its
 // state) advanced by one team of workgroups inside the persistent kernel.
@@ -103,8 +98,7 @@ struct Team {
                              // 0: rank 0 reduces and releases through gen
   int pad_;
   Slot* slots;               // [size]
-  TeamGen* gen;
-  double* scal;              // [2][kMaxRed] results published by rank 0
+  double* scal;             // [kSlotBufs][kMaxRed] tagged sums published by rank 0
   int* abort_flag;
 };
 

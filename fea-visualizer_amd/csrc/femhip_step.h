@@ -63,11 +63,11 @@ struct fem_batch {
   int* d_wg_team = nullptr;
   int* d_team_inst = nullptr;
   femhip::Slot* d_slots = nullptr;
-  femhip::TeamGen* d_gens = nullptr;
   double* d_scal = nullptr;
   int grid = 0;
   int* d_abort = nullptr;
   int nteams = 0;
+  int grid_budget = 0;          // multi-instance plans: workgroups to spread over (0 = all co-resident)
   std::vector<int> team_sizes;  // for reporting
   // plain-engine scratch
   double* d_plain = nullptr;
@@ -88,4 +88,5 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_clear(fem_batch* b, int inst);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
+int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
 }  // namespace femhip

@@ -14,15 +14,18 @@
 // (state, solution, residual, Krylov vectors, partial sums) is stored and
 // loaded device-coherently (agent-scope atomic accesses: global_load/store
 // sc1).  A barrier is then: every wave waits for its stores to complete
-// (s_waitcnt 0), s_barrier, and thread 0 publishes the workgroup's epoch in
-// its own slot with a coherent store -- the release the memory model would
-// emit for such data, without the cache-wide write-back.  Arrivals go to
-// distinct slots (no contended read-modify-write).  Read-only data (pattern,
-// matrices, preconditioner) uses ordinary cached loads.  A one-workgroup team
-// needs none of this and runs the plain variant.
+// (s_waitcnt 0 -- the release the memory model would emit for such data,
+// without the cache-wide write-back), s_barrier, and the workgroup stores its
+// partial dot products coherently into its own slot.  Each 64-bit partial
+// carries the barrier's epoch tag in its lowest mantissa bit, so the partials
+// are also the arrival signal: readers poll the values themselves, and
+// arrivals never contend on one address.  Read-only data (pattern, matrices,
+// preconditioner) uses ordinary cached loads.  A one-workgroup team needs none
+// of this and runs the plain variant.
 //
-// Reductions: partial sums are double-buffered by epoch parity in the slots
-// and always summed in rank order, so results are deterministic.
+// Reductions: partials are four-way buffered by epoch in the slots and always
+// summed in rank order, so results are deterministic and every workgroup
+// takes the same decisions.
 //
 // Per step and instance (the reference's CPUSolver::advance_time,
 // src/FEM/CPUSolver.cpp:64-147, re-expressed as fused phases):
@@ -89,12 +92,6 @@ __device__ inline void st(T* p, T v) {
   else *gp(p) = v;
 }
 
-__device__ inline unsigned uld(const unsigned* p) {
-  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void ust(unsigned* p, unsigned v) {
-  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Rank of this workgroup inside its team, ordered so that ranks on the same
 // XCD (workgroups are dealt round-robin over the 8 XCDs) own adjacent row
@@ -105,9 +102,31 @@ __device__ inline int xcd_rank(int l, int g) {
   return base + (l >> 3);
 }
 
-// Per-lane partial sums -> dst[q*K + j] (this workgroup's share).
+// Tag a partial sum for barrier epoch e: its lowest mantissa bit becomes
+// (e >> 2) & 1.  The change is at most one unit in the last place and is the
+// same on every workgroup, so reductions stay deterministic and every
+// workgroup takes the same decisions.
+__device__ inline double tag_value(double v, unsigned e) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return __longlong_as_double((long long)((b & ~1ull) | (unsigned long long)((e >> 2) & 1u)));
+}
+__device__ inline bool has_tag(double v, unsigned e) {
+  return ((unsigned long long)__double_as_longlong(v) & 1ull) == (unsigned long long)((e >> 2) & 1u);
+}
+
+// Values every barrier of a K-column instance writes per buffer.  A barrier
+// reducing fewer values pads with tagged zeros: otherwise entries last
+// written eight epochs earlier (same buffer, same tag) would look current.
+template <int K>
+constexpr int kPadVals = 3 * K;
+
+// Per-lane partial sums -> this workgroup's share at q*K + j.  A one-workgroup
+// team keeps it in LDS (sh.red).  Larger teams first wait until every wave's
+// stores of the phase are complete, then store the share tagged and coherently
+// into buffer e & 3 of their slot: the tagged values are the arrival signal.
 template <int K, int NS, bool C>
-__device__ inline void block_partials(const double (&acc)[NS], Shared& sh, double* dst) {
+__device__ inline void block_partials(const double (&acc)[NS], Shared& sh, double* dst, unsigned e) {
+  static_assert(NS * K <= kPadVals<K>, "reduction wider than the slot padding");
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < NS; q++) {
@@ -116,13 +135,21 @@ __device__ inline void block_partials(const double (&acc)[NS], Shared& sh, doubl
     for (int m = kWave / 2; m >= K; m >>= 1) v += __shfl_xor(v, m, kWave);
     if (lane < K) sh.wred[wave][q * K + lane] = v;
   }
+  if constexpr (C) __builtin_amdgcn_s_waitcnt(0);  // the phase's coherent stores have completed
   __syncthreads();
-  if (threadIdx.x < NS * K) {
+  if constexpr (C) {
+    if (threadIdx.x < kPadVals<K>) {
+      double s = 0.0;
+      if (threadIdx.x < NS * K)
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; w++) s += sh.wred[w][threadIdx.x];
+      st<true>(dst + threadIdx.x, tag_value(s, e));  // the team's slot in global memory
+    }
+  } else if (threadIdx.x < NS * K) {
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kBlock / kWave; w++) s += sh.wred[w][threadIdx.x];
-    if constexpr (C) st<true>(dst + threadIdx.x, s);  // the team's slot in global memory
-    else dst[threadIdx.x] = s;                         // sh.red in LDS
+    dst[threadIdx.x] = s;  // sh.red in LDS
   }
 }
 
@@ -130,88 +157,90 @@ __device__ inline bool aborted(const Team& T) {
   return __hip_atomic_load(T.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// Wait until every slot of the team reached epoch e.  Returns nonzero (on
-// every thread) when the wait timed out or another workgroup aborted.
-__device__ int wait_all(const Team& T, unsigned e) {
-  int to = 0;
-  for (int t = threadIdx.x; t < T.size && !to; t += kBlock) {
-    if (uld(&T.slots[t].epoch) >= e) continue;
-    const long long t0 = wall_clock64();
-    int spins = 0;
-    while (uld(&T.slots[t].epoch) < e) {
-      __builtin_amdgcn_s_sleep(1);
-      if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
-        to = 1;
-        break;
-      }
-    }
-  }
-  return __syncthreads_or(to);
-}
-
 __device__ void raise_abort(const Team& T) {
   if (threadIdx.x == 0) __hip_atomic_fetch_or(T.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sum slot partials of buffer `buf` over ranks (fixed order) into sh.red.
-__device__ void reduce_slots(const Team& T, int buf, int nvals, Shared& sh, bool publish) {
-  const int chunks = kBlock / nvals;
-  const int q = threadIdx.x % nvals, c = threadIdx.x / nvals;
-  double s = 0.0;
-  if (c < chunks)
-    for (int w = c; w < T.size; w += chunks) s += ld<true>(&T.slots[w].vals[buf][q]);
-  double* scratch = &sh.wred[0][0];  // kBlock doubles
-  if (c < chunks) scratch[c * nvals + q] = s;
-  __syncthreads();
-  if (threadIdx.x < nvals) {
-    double t = 0.0;
-    for (int cc = 0; cc < chunks; cc++) t += scratch[cc * nvals + threadIdx.x];
-    sh.red[threadIdx.x] = t;
-    if (publish) st<true>(&T.scal[buf * kMaxRed + threadIdx.x], t);
+// Re-load a tagged value until it carries epoch e's tag.  Sets *to on a
+// timeout or when another workgroup aborted.
+__device__ double wait_value(const Team& T, const double* p, unsigned e, int* to) {
+  const long long t0 = wall_clock64();
+  int spins = 0;
+  while (true) {
+    const double v = ld<true>(p);
+    if (has_tag(v, e)) return v;
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+      *to = 1;
+      return 0.0;
+    }
   }
 }
 
-// Team barrier for epoch e; the partials (if nvals > 0) are already in this
-// workgroup's slot buffer e & 1 (or in sh.red for a one-workgroup team).
-// Returns false when the launch must stop.
-__device__ bool team_sync(const Team& T, int rank, unsigned e, int nvals, Shared& sh) {
-  if (T.size == 1) {
-    __syncthreads();
-    return true;
+// Wait for the tagged partials of every rank and sum them in rank order:
+// thread (q, c) sums ranks c, c + chunks, ... with kR loads in flight, then
+// the chunk sums are combined in LDS.  publish: also store the tagged sums
+// for the other ranks (rank 0 of a large team).  Returns nonzero on abort.
+__device__ int reduce_tagged(const Team& T, unsigned e, int nvals, int npad, Shared& sh, bool publish) {
+  const int buf = e & 3;
+  const int chunks = kBlock / nvals;
+  const int q = threadIdx.x % nvals, c = threadIdx.x / nvals;
+  double s = 0.0;
+  int to = 0;
+  if (c < chunks) {
+    constexpr int kR = 8;
+    for (int w0 = c; w0 < T.size && !to; w0 += kR * chunks) {
+      double v[kR];
+#pragma unroll
+      for (int i = 0; i < kR; i++) {
+        const int w = w0 + i * chunks;
+        v[i] = ld<true>(&T.slots[w < T.size ? w : c].vals[buf][q]);
+      }
+#pragma unroll
+      for (int i = 0; i < kR; i++) {
+        const int w = w0 + i * chunks;
+        if (w < T.size && !has_tag(v[i], e)) v[i] = wait_value(T, &T.slots[w].vals[buf][q], e, &to);
+      }
+#pragma unroll
+      for (int i = 0; i < kR; i++)
+        if (w0 + i * chunks < T.size) s += v[i];
+    }
   }
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's coherent stores have completed
+  double* scratch = &sh.wred[0][0];  // kBlock doubles
+  if (c < chunks) scratch[c * nvals + q] = s;
+  if (__syncthreads_or(to)) return 1;
+  if (threadIdx.x < nvals) {
+    double t = 0.0;
+    for (int cc = 0; cc < chunks; cc++) t += scratch[cc * nvals + threadIdx.x];
+    if (publish) {
+      // every rank must use bit-identical values: rank 0 uses the tagged sum too
+      t = tag_value(t, e);
+      st<true>(&T.scal[buf * kMaxRed + threadIdx.x], t);
+    }
+    sh.red[threadIdx.x] = t;
+  } else if (publish && threadIdx.x < npad) {
+    st<true>(&T.scal[buf * kMaxRed + threadIdx.x], tag_value(0.0, e));
+  }
+  return 0;
+}
+
+// Team barrier for epoch e; this workgroup's partials are already stored.
+// Returns false when the launch must stop.
+__device__ bool team_sync(const Team& T, int rank, unsigned e, int nvals, int npad, Shared& sh) {
   __syncthreads();
-  const int buf = e & 1;
-  if (threadIdx.x == 0) ust(&T.slots[rank].epoch, e);
+  if (T.size == 1) return true;
   if (T.allpoll || rank == 0) {
-    if (wait_all(T, e)) {
+    if (reduce_tagged(T, e, nvals, npad, sh, !T.allpoll)) {
       raise_abort(T);
       return false;
     }
-    if (nvals > 0) reduce_slots(T, buf, nvals, sh, !T.allpoll);
-    if (!T.allpoll) {
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (threadIdx.x == 0) ust(&T.gen->gen, e);
-    }
   } else {
     int to = 0;
-    if (threadIdx.x == 0) {
-      const long long t0 = wall_clock64();
-      int spins = 0;
-      while (uld(&T.gen->gen) < e) {
-        __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
-          to = 1;
-          break;
-        }
-      }
-    }
+    if (threadIdx.x < nvals) sh.red[threadIdx.x] = wait_value(T, &T.scal[(e & 3) * kMaxRed + threadIdx.x], e, &to);
     if (__syncthreads_or(to)) {
       raise_abort(T);
       return false;
     }
-    if (threadIdx.x < nvals) sh.red[threadIdx.x] = ld<true>(&T.scal[buf * kMaxRed + threadIdx.x]);
   }
   __syncthreads();
   return true;
@@ -221,14 +250,17 @@ __device__ bool team_sync(const Team& T, int rank, unsigned e, int nvals, Shared
 template <int K, int NS, bool C>
 __device__ bool team_reduce(const double (&acc)[NS], const Team& T, int rank, unsigned& e, Shared& sh) {
   e++;
-  if constexpr (C) block_partials<K, NS, true>(acc, sh, T.slots[rank].vals[e & 1]);
-  else block_partials<K, NS, false>(acc, sh, sh.red);
-  return team_sync(T, rank, e, NS * K, sh);
+  // a one-workgroup team (also inside a coherent launch) reduces in LDS
+  if (!C || T.size == 1) block_partials<K, NS, false>(acc, sh, sh.red, e);
+  else block_partials<K, NS, true>(acc, sh, T.slots[rank].vals[e & 3], e);
+  return team_sync(T, rank, e, NS * K, kPadVals<K>, sh);
 }
 
+// Plain barrier: K (ignored) values per workgroup are the arrival signal.
+template <int K, bool C>
 __device__ bool team_barrier(const Team& T, int rank, unsigned& e, Shared& sh) {
-  e++;
-  return team_sync(T, rank, e, 0, sh);
+  const double acc[1] = {0.0};
+  return team_reduce<K, 1, C>(acc, T, rank, e, sh);
 }
 
 template <typename TI>
@@ -622,6 +654,210 @@ __device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, const T
   }
 }
 
+// ---------------------------------------------------------------------------
+// LDS row cache.  Rows are owned by the same lane for the whole launch, and the
+// inner-precision matrix and preconditioner do not change during it, so each
+// lane's rows are staged once per instance as (column, S_ij * D^-1_jj) pairs.
+// The Krylov gathers then have one dependent memory level (the vector loads)
+// instead of three (row pointer -> column index -> vector).  Unused entries
+// point at the row itself with weight 0, so the unrolled gather is
+// branch-free and every address is valid.
+// ---------------------------------------------------------------------------
+
+constexpr int kCacheE = 10;  // entries per cached row
+constexpr int kCacheP = 2;   // cached rows per lane
+
+template <typename TI>
+struct RowCache {
+  int col[kCacheP][kCacheE][kBlock];
+  TI val[kCacheP][kCacheE][kBlock];
+};
+
+// Fill the cache for this workgroup's rows; true (uniform) when every row fits.
+template <int K, typename TI>
+__device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, RowCache<TI>& RC) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  const TI GA* S = gp((const TI*)I.Si);
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  int ok = (row_hi - row_lo + RPP - 1) / RPP <= kCacheP;
+  if (ok) {
+    for (int q = 0; q < kCacheP; q++) {
+      const int row = row_lo + (int)threadIdx.x / K + q * RPP;
+      int lo = 0, hi = 0;
+      if (row < row_hi) {
+        lo = rp[row];
+        hi = rp[row + 1];
+        if (hi - lo > kCacheE) ok = 0;
+      }
+      for (int e = 0; e < kCacheE; e++) {
+        const int s = lo + e;
+        const bool v = s < hi;
+        const int c = v ? ci[s] : (row < row_hi ? row : 0);
+        RC.col[q][e][threadIdx.x] = c;
+        RC.val[q][e][threadIdx.x] = v ? S[(size_t)s * kv + vj] * dinv[(size_t)c * kv + vj] : (TI)0;
+      }
+    }
+  }
+  return __syncthreads_and(ok) != 0;
+}
+
+// Pipelined CG start from the cache (same arithmetic as phase_pcg_init).
+template <int K, typename TI, bool C>
+__device__ void phase_pcg_init_c(const Inst& I, int row_lo, int row_hi, const TI* rc, TI* wc,
+                                 const RowCache<TI>& RC, double (&acc)[2]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  TI* xi = (TI*)I.xi;
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  for (int q = 0; q < kCacheP; q++) {
+    const int row = row_lo + (int)threadIdx.x / K + q * RPP;
+    if (row >= row_hi) break;
+    const size_t o = (size_t)row * K + jr;
+    const double xo = ld<C>(I.x + o), xio = (double)ld<C>(xi + o);
+    const TI ro = ld<C>(rc + o);
+    TI g[kCacheE];
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) g[e] = ld<C>(rc + (size_t)RC.col[q][e][threadIdx.x] * K + jr);
+    TI wa = (TI)0;
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) wa += RC.val[q][e][threadIdx.x] * g[e];
+    st<C>(I.x + o, xo + xio);
+    st<C>(xi + o, (TI)0);
+    st<C>(wc + o, wa);
+    const TI uo = dinv[(size_t)row * kv + vj] * ro;
+    acc[0] += (double)ro * (double)uo;
+    acc[1] += (double)uo * (double)wa;
+  }
+}
+
+// One Chronopoulos-Gear iteration from the cache (same arithmetic as phase_pcg_iter).
+template <int K, typename TI, bool C>
+__device__ void phase_pcg_iter_c(const Inst& I, int row_lo, int row_hi, const TI* rc, const TI* wc, const TI* sc,
+                                 TI* rn, TI* wn, TI* sn, bool first, const Shared& sh, const RowCache<TI>& RC,
+                                 double (&acc)[3]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  TI* xi = (TI*)I.xi;
+  TI GA* p = gp((TI*)I.p0);  // own rows only
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const TI al = to_ti<TI>(sh.alpha[jr]);
+  const TI be = first ? (TI)0 : to_ti<TI>(sh.beta[jr]);
+  for (int q = 0; q < kCacheP; q++) {
+    const int row = row_lo + (int)threadIdx.x / K + q * RPP;
+    if (row >= row_hi) break;
+    const size_t o = (size_t)row * K + jr;
+    const TI ro = ld<C>(rc + o);
+    const TI wo = ld<C>(wc + o);
+    const TI so = first ? (TI)0 : ld<C>(sc + o);
+    const TI po = first ? (TI)0 : p[o];
+    const TI xio = ld<C>(xi + o);
+    TI gr[kCacheE], gw[kCacheE], gs[kCacheE];
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) {
+      const size_t oc = (size_t)RC.col[q][e][threadIdx.x] * K + jr;
+      gr[e] = ld<C>(rc + oc);
+      gw[e] = ld<C>(wc + oc);
+      gs[e] = first ? (TI)0 : ld<C>(sc + oc);
+    }
+    TI wa = (TI)0;
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++)
+      wa += RC.val[q][e][threadIdx.x] * cg_r<TI>(gr[e], cg_s<TI>(gw[e], gs[e], be, first), al);
+    const TI d = dinv[(size_t)row * kv + vj];
+    const TI uo = d * ro;
+    const TI pn = first ? uo : (uo + be * po);
+    const TI sv = cg_s<TI>(wo, so, be, first);
+    const TI rr = cg_r<TI>(ro, sv, al);
+    p[o] = pn;
+    st<C>(xi + o, xio + al * pn);
+    st<C>(sn + o, sv);
+    st<C>(rn + o, rr);
+    st<C>(wn + o, wa);
+    const TI un = d * rr;
+    acc[0] += (double)rr * (double)un;
+    acc[1] += (double)un * (double)wa;
+    acc[2] += (double)rr * (double)rr;
+  }
+}
+
+// BiCGSTAB phase P from the cache.
+template <int K, typename TI, bool C>
+__device__ void phase_bi_p_c(const Inst& I, int row_lo, int row_hi, const TI* r, const TI* pold, TI* pnew,
+                             const TI* vold, TI* vnew, bool first, const Shared& sh, const RowCache<TI>& RC,
+                             double (&acc)[1]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  TI GA* rh = gp((TI*)I.rh);  // own rows only
+  TI* xi = (TI*)I.xi;
+  const TI be = first ? (TI)0 : to_ti<TI>(sh.beta[jr]);
+  const TI om = first ? (TI)0 : to_ti<TI>(sh.omega[jr]);
+  for (int q = 0; q < kCacheP; q++) {
+    const int row = row_lo + (int)threadIdx.x / K + q * RPP;
+    if (row >= row_hi) break;
+    const size_t o = (size_t)row * K + jr;
+    const TI ro = ld<C>(r + o);
+    if (first) {
+      st<C>(I.x + o, ld<C>(I.x + o) + (double)ld<C>(xi + o));
+      st<C>(xi + o, (TI)0);
+      rh[o] = ro;
+    }
+    TI gr[kCacheE], gp_[kCacheE], gv[kCacheE];
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) {
+      const size_t oc = (size_t)RC.col[q][e][threadIdx.x] * K + jr;
+      gr[e] = ld<C>(r + oc);
+      gp_[e] = first ? (TI)0 : ld<C>(pold + oc);
+      gv[e] = first ? (TI)0 : ld<C>(vold + oc);
+    }
+    TI va = (TI)0;
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++)
+      va += RC.val[q][e][threadIdx.x] * (first ? gr[e] : (gr[e] + be * (gp_[e] - om * gv[e])));
+    const TI pn = first ? ro : (ro + be * (ld<C>(pold + o) - om * ld<C>(vold + o)));
+    st<C>(pnew + o, pn);
+    st<C>(vnew + o, va);
+    acc[0] += (double)(first ? ro : rh[o]) * (double)va;
+  }
+}
+
+// BiCGSTAB phase S from the cache.
+template <int K, typename TI, bool C>
+__device__ void phase_bi_s_c(const Inst& I, int row_lo, int row_hi, const TI* r, const TI* vnew, const Shared& sh,
+                             const RowCache<TI>& RC, double (&acc)[3]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  TI GA* sv = gp((TI*)I.s);  // own rows only
+  TI GA* tv = gp((TI*)I.t);  // own rows only
+  const TI al = to_ti<TI>(sh.alpha[jr]);
+  for (int q = 0; q < kCacheP; q++) {
+    const int row = row_lo + (int)threadIdx.x / K + q * RPP;
+    if (row >= row_hi) break;
+    const size_t o = (size_t)row * K + jr;
+    const TI so = ld<C>(r + o) - al * ld<C>(vnew + o);
+    TI gr[kCacheE], gv[kCacheE];
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) {
+      const size_t oc = (size_t)RC.col[q][e][threadIdx.x] * K + jr;
+      gr[e] = ld<C>(r + oc);
+      gv[e] = ld<C>(vnew + oc);
+    }
+    TI ta = (TI)0;
+#pragma unroll
+    for (int e = 0; e < kCacheE; e++) ta += RC.val[q][e][threadIdx.x] * (gr[e] - al * gv[e]);
+    sv[o] = so;
+    tv[o] = ta;
+    acc[0] += (double)ta * (double)so;
+    acc[1] += (double)ta * (double)ta;
+    acc[2] += (double)so * (double)so;
+  }
+}
+
 // State update and the instability rule (CPUSolver.cpp:41-48, widened to
 // flag NaN/Inf separately).
 template <int K, typename TI, bool C>
@@ -677,14 +913,15 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
 // Pipelined (Chronopoulos-Gear) PCG on the current sweep; r starts in I.r.
 template <int K, typename TI, bool C>
 __device__ bool krylov_pcg(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo, int row_hi,
-                           int& its) {
+                           int& its, const RowCache<TI>& RC, bool cached) {
   TI* rb[2] = {(TI*)I.r, (TI*)I.z};
   TI* wb[2] = {(TI*)I.q, (TI*)I.rh};
   TI* sb[2] = {(TI*)I.s, (TI*)I.t};
   int par = 0;
   {
     double acc[2] = {0.0, 0.0};
-    phase_pcg_init<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], acc);
+    if (cached) phase_pcg_init_c<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], RC, acc);
+    else phase_pcg_init<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], acc);
     if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
   }
   if (threadIdx.x < K) {
@@ -700,8 +937,12 @@ __device__ bool krylov_pcg(const Inst& I, const Team& T, int rank, unsigned& e, 
   while (true) {
     {
       double acc[3] = {0.0, 0.0, 0.0};
-      phase_pcg_iter<K, TI, C>(I, row_lo, row_hi, rb[par], wb[par], sb[par], rb[par ^ 1], wb[par ^ 1], sb[par ^ 1],
-                               first, sh, acc);
+      if (cached)
+        phase_pcg_iter_c<K, TI, C>(I, row_lo, row_hi, rb[par], wb[par], sb[par], rb[par ^ 1], wb[par ^ 1],
+                                   sb[par ^ 1], first, sh, RC, acc);
+      else
+        phase_pcg_iter<K, TI, C>(I, row_lo, row_hi, rb[par], wb[par], sb[par], rb[par ^ 1], wb[par ^ 1], sb[par ^ 1],
+                                 first, sh, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
     }
     par ^= 1;
@@ -786,7 +1027,7 @@ __device__ bool krylov_cg_classic(const Inst& I, const Team& T, int rank, unsign
 
 template <int K, typename TI, bool C>
 __device__ bool krylov_bicgstab(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo,
-                                int row_hi, int& its) {
+                                int row_hi, int& its, const RowCache<TI>& RC, bool cached) {
   TI* r = (TI*)I.r;
   TI* pa = (TI*)I.p0;
   TI* pb = (TI*)I.p1b;
@@ -797,7 +1038,8 @@ __device__ bool krylov_bicgstab(const Inst& I, const Team& T, int rank, unsigned
   while (true) {
     {
       double acc[1] = {0.0};
-      phase_bi_p<K, TI, C>(I, row_lo, row_hi, r, pa, pb, va, vb, first, sh, acc);
+      if (cached) phase_bi_p_c<K, TI, C>(I, row_lo, row_hi, r, pa, pb, va, vb, first, sh, RC, acc);
+      else phase_bi_p<K, TI, C>(I, row_lo, row_hi, r, pa, pb, va, vb, first, sh, acc);
       if (!team_reduce<K, 1, C>(acc, T, rank, e, sh)) return false;
     }
     if (threadIdx.x < K) {
@@ -807,7 +1049,8 @@ __device__ bool krylov_bicgstab(const Inst& I, const Team& T, int rank, unsigned
     __syncthreads();
     {
       double acc[3] = {0.0, 0.0, 0.0};
-      phase_bi_s<K, TI, C>(I, row_lo, row_hi, r, vb, sh, acc);
+      if (cached) phase_bi_s_c<K, TI, C>(I, row_lo, row_hi, r, vb, sh, RC, acc);
+      else phase_bi_s<K, TI, C>(I, row_lo, row_hi, r, vb, sh, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
     }
     if (threadIdx.x < K) {
@@ -856,11 +1099,14 @@ __device__ bool krylov_bicgstab(const Inst& I, const Team& T, int rank, unsigned
 // 2 classic CG (one Krylov method per kernel keeps register pressure down).
 // Returns false when the launch aborts.
 template <int K, typename TI, int KR, bool C>
-__device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int nsteps) {
+__device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, RowCache<TI>& RC,
+                             int nsteps) {
   const int n = I.n;
   const int rows_per_wg = (n + T.size - 1) / T.size;
   const int row_lo = min(n, rank * rows_per_wg);
   const int row_hi = min(n, row_lo + rows_per_wg);
+  bool cached = false;
+  if constexpr (KR != 2) cached = fill_cache<K, TI>(I, row_lo, row_hi, RC);
   long long its_total = 0, sweeps_total = 0;
   int status = 0;
   TI* r0 = (TI*)I.r;
@@ -898,9 +1144,9 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       }
       __syncthreads();
       bool ok;
-      if constexpr (KR == 1) ok = krylov_bicgstab<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
+      if constexpr (KR == 1) ok = krylov_bicgstab<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached);
       else if constexpr (KR == 2) ok = krylov_cg_classic<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
-      else ok = krylov_pcg<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
+      else ok = krylov_pcg<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached);
       if (!ok) return false;
       // ---- Res: FP64 residual of x + xi
       {
@@ -929,7 +1175,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     its_total += its;
     sweeps_total += sweeps;
     // the next step (or the next instance of this team) gathers the new state
-    if (!team_barrier(T, rank, e, sh)) return false;
+    if (!team_barrier<K, C>(T, rank, e, sh)) return false;
   }
   if (rank == 0 && threadIdx.x == 0) {
     I.counters[0] += its_total;
@@ -948,6 +1194,7 @@ template <int K, typename TI, int KR, bool C>
 __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* wg_team, const Inst* insts,
                                                 const int* team_inst, int nsteps) {
   __shared__ Shared sh;
+  __shared__ RowCache<TI> RC;
   const int tid_team = wg_team[blockIdx.x];
   if (tid_team < 0) return;
   const Team T = teams[tid_team];
@@ -957,7 +1204,7 @@ __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* w
   for (int qi = T.inst_begin; qi < T.inst_end; qi++) {
     const Inst& I = insts[team_inst[qi]];
     if (I.n <= 0) continue;
-    if (!run_instance<K, TI, KR, C>(I, T, rank, e, sh, nsteps)) return;
+    if (!run_instance<K, TI, KR, C>(I, T, rank, e, sh, RC, nsteps)) return;
   }
 }
 

@@ -165,6 +165,7 @@ SIGNATURES = [
     ("fem_solver_unstable", C.c_int, [P, PI]),
     ("fem_solver_state_device_ptr", C.c_int, [P, I32, C.POINTER(P)]),
     ("fem_solver_sync", C.c_int, [P]),
+    ("fem_solver_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_batch_create", C.c_int, [C.POINTER(P), I32, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_batch_destroy", C.c_int, [P]),
     ("fem_batch_set_equation", C.c_int, [P, I32, I32, P, C.c_size_t]),
@@ -172,6 +173,7 @@ SIGNATURES = [
     ("fem_batch_get_state_unknowns", C.c_int, [P, I32, I32, PD]),
     ("fem_batch_advance", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_sync", C.c_int, [P]),
+    ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),
 ]
 
 _lib = None

@@ -281,6 +281,15 @@ class HIPSolver:
     def sync(self):
         L.check(self._lib.fem_solver_sync(self.handle))
 
+    def autotune(self, probe_steps: int = 20):
+        """Pick the team size by timing probe_steps under a few launch plans
+        (the state is restored); returns (team_size, best probe ms)."""
+        c = C.c_int32()
+        ms = C.c_double()
+        L.check(self._lib.fem_solver_autotune(self.handle, int(probe_steps), C.byref(c), C.byref(ms)))
+        self.options.team_size = c.value
+        return c.value, ms.value
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.fem_solver_destroy(self.handle)
@@ -328,6 +337,14 @@ class Batch:
 
     def sync(self):
         L.check(self._lib.fem_batch_sync(self.handle))
+
+    def autotune(self, probe_steps: int = 5):
+        """Pick the workgroup budget of the team plan (state restored);
+        returns (budget, best probe ms)."""
+        c = C.c_int32()
+        ms = C.c_double()
+        L.check(self._lib.fem_batch_autotune(self.handle, int(probe_steps), C.byref(c), C.byref(ms)))
+        return c.value, ms.value
 
     def close(self):
         if getattr(self, "handle", None):

@@ -179,6 +179,10 @@ int fem_solver_unstable(fem_solver* s, int32_t* unstable);
 int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr);
 /* Synchronise the solver stream. */
 int fem_solver_sync(fem_solver* s);
+/* Autotuner (no reference counterpart): runs probe_steps under a few launch
+ * plans from the current state, restores the state after each, and keeps the
+ * fastest.  *chosen receives the team size (workgroups per instance). */
+int fem_solver_autotune(fem_solver* s, int32_t probe_steps, int32_t* chosen, double* best_ms);
 
 /* ---- batch: many contexts (meshes of uneven size) advanced in one
  * persistent launch; no reference counterpart ---- */
@@ -189,6 +193,9 @@ int fem_batch_set_state_unknowns(fem_batch* b, int32_t instance, int32_t field, 
 int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, double* x);
 int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats);
 int fem_batch_sync(fem_batch* b);
+/* Autotuner over the number of workgroups the batch plan spreads over
+ * (*chosen); state restored as for fem_solver_autotune. */
+int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, double* best_ms);
 
 #ifdef __cplusplus
 }

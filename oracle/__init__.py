@@ -204,15 +204,25 @@ def csr_to_csc(n, rp, ci, vals):
     return (m.indptr.astype(np.int32), m.indices.astype(np.int32), np.ascontiguousarray(m.data))
 
 
-def port_run(sysf: System, equation, prm, nsteps, u, v=None):
+def port_prepare(sysf: System):
+    """CSC float32 copies of K, M, A (FEMContext's Eigen storage order)."""
+    n = sysf.n
+    return {
+        "n": n,
+        "K": csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.K.astype(np.float32)),
+        "M": csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.M.astype(np.float32)),
+        "A": csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.A.astype(np.float32)),
+    }
+
+
+def port_run(sysf, equation, prm, nsteps, u, v=None):
     """Reference-cost FP32 port (CPUSolver semantics, Eigen CG defaults).
 
-    Advances nsteps in place on float32 copies; returns (iterations, u, v)."""
+    sysf: a System or the result of port_prepare.  Advances nsteps on float32
+    copies; returns (iterations, u, v)."""
     lib = load()
-    n = sysf.n
-    K = csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.K.astype(np.float32))
-    M = csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.M.astype(np.float32))
-    A = csr_to_csc(n, sysf.row_ptr, sysf.col_idx, sysf.A.astype(np.float32))
+    prep = sysf if isinstance(sysf, dict) else port_prepare(sysf)
+    n, K, M, A = prep["n"], prep["K"], prep["M"], prep["A"]
     u = np.ascontiguousarray(u, dtype=np.float32).copy()
     v = np.zeros_like(u) if v is None else np.ascontiguousarray(v, dtype=np.float32).copy()
     pr = np.ascontiguousarray(prm, dtype=np.float64)

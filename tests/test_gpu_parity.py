@@ -245,3 +245,46 @@ def test_deterministic_bitwise(gpu):
         sol.advance_time(5)
         out.append(sol.get_state_unknowns())
     assert np.array_equal(out[0], out[1])
+
+
+def test_batch_mixed_team_sizes(gpu):
+    """One large and several tiny meshes: the plan mixes multi-workgroup teams
+    with single-workgroup teams that each walk several instances."""
+    sides = [64, 4, 5, 6, 4, 5, 6, 7]
+    ctxs, systems, states = [], [], []
+    rng = np.random.default_rng(9)
+    for i, side in enumerate(sides):
+        m = meshes.jittered_square(side, seed=20 + i)
+        ctxs.append(FEMContext(m, 0))
+        s = oracle.System(m, bc=0)
+        systems.append(s)
+        states.append(rng.normal(size=s.n))
+    b = Batch(ctxs, SolverOptions())
+    for i in range(len(sides)):
+        b.set_equation(i, HeatParameters(2e-3, 0.05 * (i + 1)))
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+    assert b.advance(4) == _lib.FEM_OK
+    for i, s in enumerate(systems):
+        _, u, _, _ = s.run(oracle.HEAT, [2e-3, 0.05 * (i + 1)], 4, states[i].copy(), tol=1e-14)
+        assert rel(b.get_state_unknowns(i), u) < RTOL, i
+
+
+@pytest.mark.parametrize("team", [3, 37, 400])
+def test_team_sizes_and_reduction_modes(gpu, team):
+    """Explicit team sizes, including one large enough that rank 0 reduces and
+    publishes the sums (Gray-Scott: 2 columns x 3 partials x 400 > 2048)."""
+    m = meshes.icosphere(4)
+    s = oracle.System(m, bc=1)
+    rng = np.random.default_rng(2)
+    u = np.ones(s.n)
+    v = np.where(rng.uniform(size=s.n) < 0.05, 0.25, 0.0)
+    prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    ctx = FEMContext(m, 1)
+    sol = make_solver(ctx, Equation.Reaction_Diffusion, prm, team_size=team)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, v)
+    for _ in range(6):
+        _, u, v, _ = s.step(oracle.GRAYSCOTT, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], u, v, tol=1e-14)
+    assert sol.advance_time(6) == _lib.FEM_OK
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
