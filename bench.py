@@ -94,6 +94,117 @@ def run_cpu_baseline(mesh, u0, v0, prm, nsteps):
     }
 
 
+def spmv_measure(side=4096, reps=50):
+    """Stand-alone CSR SpMV/SpMM (the product inside every CG iteration) on a
+    synthetic CSR larger than the 256 MB Infinity Cache: the jittered
+    side x side lattice (Delaunay split, Dirichlet), S = 1000 M + K."""
+    import torch
+    from femhip import FEMContext, Operator, _lib, meshes
+    mesh = meshes.jittered_square(side, seed=2)
+    ctx = FEMContext(mesh, 0)
+    n = ctx.num_unknowns()
+    nnz = ctx.info().nnz
+    out = {"matrix": f"jittered {side}x{side} lattice, Delaunay split, Dirichlet: N={n}, nnz={nnz}, "
+                     f"int32 columns, S = 1000 M + K",
+           "bytes_formula": "(s_val + 4) nnz + 4 (N + 1) + 2 k N s_val", "results": []}
+    for prec, vb, tdt in ((_lib.FEM_PREC_FP32_IR, 4, torch.float32), (_lib.FEM_PREC_FP64, 8, torch.float64)):
+        op = Operator(ctx, (1000.0, 1.0, 0.0), prec)
+        for k in (1, 16):
+            if vb == 8 and k == 16:
+                continue
+            x = torch.randn(n * k, dtype=tdt, device="cuda")
+            y = torch.empty_like(x)
+            torch.cuda.synchronize()
+            sched, _ = op.autotune(k, x.data_ptr(), y.data_ptr(), reps=5)
+            ms = op.time(k, x.data_ptr(), y.data_ptr(), reps=reps)
+            b = (vb + 4) * nnz + 4 * (n + 1) + 2 * k * n * vb
+            gbs = b / (ms * 1e-3) / 1e9
+            out["results"].append({"values": "f32" if vb == 4 else "f64", "nrhs": k, "schedule": sched,
+                                   "ms": ms, "bytes": b, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
+            del x, y
+        op.close()
+    ctx.close()
+    return out
+
+
+def adjoint_measure(side=512, nsteps=100, every=10, fd_dirs=3):
+    """Differentiable advection-diffusion: jittered side^2 lattice, rotating
+    velocity (-(y-1/2), x-1/2) at element centroids (unnormalised), per-element
+    diffusivity 1e-3 exp(0.3 z), dt = 2e-3, nsteps steps, L = |u_T - y|^2;
+    adjoint with a checkpoint every `every` steps.  Timed: one forward and one
+    backward of the FP32 + FP64-refinement solver (wall clock, inputs on the
+    host as the C ABI takes them).  Accuracy: gradients against the FP64
+    solver at tol 1e-13, and central differences (h = 1e-6, FP64) along
+    random directions."""
+    from femhip import Adjoint, AdvectionDiffusionParameters, FEMContext, SolverOptions, _lib, meshes
+    mesh = meshes.jittered_square(side, seed=6)
+    cen = mesh["xyz"].astype(np.float64)[mesh["tri"]].mean(axis=1)
+    ve = np.stack([-(cen[:, 1] - 0.5), cen[:, 0] - 0.5, np.zeros(len(cen))], axis=1)
+    rk = np.random.Generator(np.random.PCG64(7))
+    kappa = 1e-3 * np.exp(0.3 * rk.standard_normal(mesh["tri"].shape[0]))
+    ctx = FEMContext(mesh, 0, velocity_e=ve, normalize_velocity=False, kappa_e=kappa)
+    n = ctx.num_unknowns()
+    X = mesh["xyz"].astype(np.float64)
+    keep = ctx.idx_map >= 0
+    u0 = np.exp(-((X[:, 0] - 0.3) ** 2 + (X[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))[keep]
+    rt = np.random.Generator(np.random.PCG64(8))
+    cc = rt.uniform(0.2, 0.8, 2)
+    target = np.exp(-((X[:, 0] - cc[0]) ** 2 + (X[:, 1] - cc[1]) ** 2) / (2 * 0.05 ** 2))[keep]
+    prm = AdvectionDiffusionParameters(2e-3, 1.0)
+    res = {}
+    timing = None
+    for name, opts in (("fp32ir", SolverOptions(tol=1e-10)),
+                       ("fp64", SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-13))):
+        adj = Adjoint(ctx, prm, opts)
+        adj.forward(u0, nsteps, every)  # warm
+        t = time.perf_counter()
+        uT = adj.forward(u0, nsteps, every)
+        tf = time.perf_counter() - t
+        t = time.perf_counter()
+        gu, gk = adj.backward(2.0 * (uT - target))
+        tb = time.perf_counter() - t
+        res[name] = (uT, gu, gk, adj)
+        if name == "fp32ir":
+            timing = {"forward_ms": 1e3 * tf, "backward_ms": 1e3 * tb, "adjoint_over_forward": tb / tf,
+                      "forward_iterations": int(adj.forward_stats.iterations),
+                      "backward_iterations": int(adj.backward_stats.iterations),
+                      "forward_kernel_ms": adj.forward_stats.ms}
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    uT64, gu64, gk64, adj64 = res["fp64"]
+    out = {"workload": f"advection-diffusion, jittered {side}x{side} lattice (N={n}, {mesh['tri'].shape[0]} "
+                       f"triangles), rotating velocity, kappa_e = 1e-3 exp(0.3 z), dt=2e-3, {nsteps} steps, "
+                       f"checkpoint every {every}",
+           **timing,
+           "rel_l2_vs_fp64": {"u_T": rel(res["fp32ir"][0], uT64), "grad_u0": rel(res["fp32ir"][1], gu64),
+                              "grad_kappa": rel(res["fp32ir"][2], gk64)}}
+    # central differences of the FP64 forward along random directions
+    rd = np.random.Generator(np.random.PCG64(9))
+    h = 1e-6
+    fd = []
+
+    def loss(u, k=None):
+        if k is not None:
+            ctx.assemble_matrices(velocity_e=ve, normalize_velocity=False, kappa_e=k)
+        v = adj64.forward(u, nsteps, every)
+        return float(np.sum((v - target) ** 2))
+    for _ in range(fd_dirs):
+        d = rd.standard_normal(n)
+        f = (loss(u0 + h * d) - loss(u0 - h * d)) / (2 * h)
+        fd.append({"wrt": "u0", "fd": f, "adjoint": float(gu64 @ d), "rel": abs(f - gu64 @ d) / abs(f)})
+        dk = rd.standard_normal(kappa.shape[0]) * kappa
+        f = (loss(u0, kappa + h * dk) - loss(u0, kappa - h * dk)) / (2 * h)
+        fd.append({"wrt": "kappa", "fd": f, "adjoint": float(gk64 @ dk), "rel": abs(f - gk64 @ dk) / abs(f)})
+    ctx.assemble_matrices(velocity_e=ve, normalize_velocity=False, kappa_e=kappa)
+    out["central_differences_fp64"] = fd
+    for v in res.values():
+        v[3].close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +220,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-spmv", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (counter calibration)")
     args = ap.parse_args()
@@ -240,6 +353,11 @@ def main():
                                  "iterations": int(pl.last_stats.iterations),
                                  "speedup_of_value": value / pv}
         pl.close()
+
+    if not args.no_spmv:
+        out["spmv"] = spmv_measure()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_measure()
 
     if not args.no_cpu and args.cpu_steps > 0:
         out["cpu_baseline"] = run_cpu_baseline(mesh, u0, v0, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], args.cpu_steps)

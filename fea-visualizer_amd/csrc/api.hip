@@ -1,7 +1,7 @@
 // api.hip -- the extern "C" boundary (include/femhip.h).  Every entry point
 // validates its arguments, converts C++ failures into status codes and
 // records a message for fem_last_error(); no exception crosses the ABI.
-#include "femhip_step.h"
+#include "femhip_adjoint.h"
 
 #include <cstdio>
 #include <cstring>
@@ -399,6 +399,142 @@ int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, doubl
     const int rc = batch_autotune(b, probe_steps, &c, best_ms);
     if (chosen) *chosen = c;
     return rc;
+  })
+}
+
+// ----------------------------------------------------------------- operator
+
+int fem_operator_create(fem_ctx* ctx, const double coef[3], int32_t precision, int32_t transpose,
+                        fem_operator** out) {
+  FEM_GUARD({
+    if (!ctx || !coef || !out) {
+      set_error("fem_operator_create: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return operator_create(ctx, coef, precision, transpose ? 1 : 0, out);
+  })
+}
+
+int fem_operator_destroy(fem_operator* op) {
+  FEM_GUARD({
+    operator_free(op);
+    return FEM_OK;
+  })
+}
+
+int fem_operator_apply_device(fem_operator* op, int32_t nrhs, const void* x, void* y) {
+  FEM_GUARD({
+    if (!op || (op->n > 0 && (!x || !y))) {
+      set_error("fem_operator_apply_device: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return operator_apply_device(op, nrhs, x, y);
+  })
+}
+
+int fem_operator_apply(fem_operator* op, int32_t nrhs, const double* x, double* y) {
+  FEM_GUARD({
+    if (!op || (op->n > 0 && (!x || !y))) {
+      set_error("fem_operator_apply: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return operator_apply_host(op, nrhs, x, y);
+  })
+}
+
+int fem_operator_time(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps, double* ms) {
+  FEM_GUARD({
+    if (!op || !ms || (op->n > 0 && (!x || !y))) {
+      set_error("fem_operator_time: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return operator_time(op, nrhs, x, y, reps, ms);
+  })
+}
+
+int fem_operator_autotune(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps, int32_t* lanes,
+                          double* ms) {
+  FEM_GUARD({
+    if (!op || (op->n > 0 && (!x || !y))) {
+      set_error("fem_operator_autotune: null argument");
+      return FEM_ERR_INVALID;
+    }
+    int g = 0;
+    const int rc = operator_autotune(op, nrhs, x, y, reps, &g, ms);
+    if (lanes) *lanes = g;
+    return rc;
+  })
+}
+
+int fem_operator_set_schedule(fem_operator* op, int32_t nrhs, int32_t schedule) {
+  FEM_GUARD({
+    if (!op) {
+      set_error("null operator");
+      return FEM_ERR_INVALID;
+    }
+    return operator_set_lanes(op, nrhs, schedule);
+  })
+}
+
+int fem_operator_sync(fem_operator* op) {
+  FEM_GUARD({
+    if (!op) {
+      set_error("null operator");
+      return FEM_ERR_INVALID;
+    }
+    FEM_HIP(hipStreamSynchronize(op->stream));
+    return FEM_OK;
+  })
+}
+
+// ------------------------------------------------------------------ adjoint
+
+int fem_adjoint_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_adjoint** out) {
+  FEM_GUARD({
+    if (!ctx || !out) {
+      set_error("fem_adjoint_create: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_create(ctx, opts, out);
+  })
+}
+
+int fem_adjoint_destroy(fem_adjoint* a) {
+  FEM_GUARD({
+    adjoint_free(a);
+    return FEM_OK;
+  })
+}
+
+int fem_adjoint_set_equation(fem_adjoint* a, int32_t equation, const void* params, size_t size) {
+  FEM_GUARD({
+    if (!a || !params) {
+      set_error("fem_adjoint_set_equation: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_set_equation(a, equation, params, size);
+  })
+}
+
+int fem_adjoint_forward(fem_adjoint* a, const double* u0, int32_t nsteps, int32_t checkpoint_every, double* uT,
+                        fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!a) {
+      set_error("null adjoint");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_forward(a, u0, nsteps, checkpoint_every, uT, stats);
+  })
+}
+
+int fem_adjoint_backward(fem_adjoint* a, const double* grad_uT, double* grad_u0, double* grad_kappa_e,
+                         fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!a) {
+      set_error("null adjoint");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_backward(a, grad_uT, grad_u0, grad_kappa_e, stats);
   })
 }
 

@@ -21,13 +21,15 @@ struct Coef {
   double cM[kMaxK], cK[kMaxK], cA[kMaxK];
 };
 
+// tperm (optional) forms the transpose: entry (r, c) takes the values of (c, r).
 template <typename TI>
-__global__ void k_form(int n, const int* rp, const int* diag, const double* M, const double* Kk, const double* A,
-                       int kv, Coef cf, double* S64, TI* Si, double* dinv64, TI* dinvi) {
+__global__ void k_form(int n, const int* rp, const int* diag, const int* tperm, const double* M, const double* Kk,
+                       const double* A, int kv, Coef cf, double* S64, TI* Si, double* dinv64, TI* dinvi) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   for (int s = rp[r]; s < rp[r + 1]; s++) {
-    const double m = M[s], kk = Kk[s], a = A[s];
+    const int t = tperm ? tperm[s] : s;
+    const double m = M[t], kk = Kk[t], a = A[t];
     for (int j = 0; j < kv; j++) {
       double val = 0.0;
       if (cf.cM[j] != 0.0) val += cf.cM[j] * m;
@@ -92,8 +94,10 @@ void free_inst(InstHost& h) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   fem_ctx* c = h.ctx;
+  const int tr = h.transpose;
   h = InstHost();
   h.ctx = c;
+  h.transpose = tr;
 }
 
 size_t inner_bytes(const fem_batch* b) { return b->opts.precision == FEM_PREC_FP32_IR ? 4 : 8; }
@@ -167,13 +171,14 @@ int form_inst(fem_batch* b, InstHost& h) {
       return FEM_ERR_STATE;
   }
   if (c->n > 0) {
+    const int* tp = h.transpose ? c->d_tperm : nullptr;
     if (inner_bytes(b) == 4)
       hipLaunchKernelGGL(k_form<float>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n, c->d_rp,
-                         c->d_diag, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (float*)h.d_Si, h.d_dinv64,
+                         c->d_diag, tp, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (float*)h.d_Si, h.d_dinv64,
                          (float*)h.d_dinvi);
     else
       hipLaunchKernelGGL(k_form<double>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n, c->d_rp,
-                         c->d_diag, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (double*)h.d_Si, h.d_dinv64,
+                         c->d_diag, tp, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (double*)h.d_Si, h.d_dinv64,
                          (double*)h.d_dinvi);
     FEM_HIP(hipGetLastError());
   }
@@ -191,7 +196,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.nnz = (int)c->nnz;
   I.k = h.k;
   I.kv = h.kv;
-  I.eq = h.eq;
+  I.eq = (h.eq_override >= 0) ? h.eq_override : h.eq;
   int kr = o.krylov;
   if (kr == FEM_KRYLOV_AUTO) kr = (h.eq == FEM_ADVECTION_DIFFUSION) ? FEM_KRYLOV_BICGSTAB : FEM_KRYLOV_CG;
   I.krylov = (kr == FEM_KRYLOV_BICGSTAB) ? 1 : (kr == FEM_KRYLOV_CG_CLASSIC) ? 2 : 0;
@@ -230,6 +235,8 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.status = h.d_status;
   I.counters = h.d_counters;
   I.last_res = h.d_last_res;
+  I.traj = h.traj;
+  I.traj_every = h.traj ? std::max(1, h.traj_every) : 1;
 }
 
 void free_plan(fem_batch* b) {
@@ -624,7 +631,14 @@ int batch_prepare(fem_batch* b, bool need_plan) {
     }
     fill_dev(b, h);
   }
-  if (need_plan && !b->plan_ready) return build_plan(b);
+  if (!need_plan) return 0;
+  if (!b->plan_ready) return build_plan(b);
+  // descriptors may have changed since the plan was built (trajectory
+  // records, equation override): upload the current ones
+  std::vector<Inst> hi(b->insts.size());
+  for (size_t i = 0; i < hi.size(); i++) hi[i] = b->insts[i].dev;
+  FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * hi.size(), hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
 }
 

@@ -30,8 +30,31 @@ struct fem_ctx {
   std::vector<int> h_rp;
 };
 
+struct fem_operator {
+  fem_ctx* ctx = nullptr;
+  int n = 0;
+  long long nnz = 0;
+  int precision = FEM_PREC_FP32_IR;  // FP32 values and vectors, or FP64
+  void* d_val = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int lanes[17] = {0};         // chosen row-group width per column count (0 = default)
+  double* d_host_x = nullptr;  // staging for the host convenience path
+  double* d_host_y = nullptr;
+  long long host_cap = 0;
+};
+
 namespace femhip {
 int ctx_create(const fem_mesh* mesh, int bc, fem_ctx** out);
 int ctx_assemble(fem_ctx* c, const fem_assembly_params* prm);
 void ctx_free(fem_ctx* c);
+
+// stand-alone CSR operator (operator.hip)
+int operator_create(fem_ctx* c, const double coef[3], int precision, int transpose, fem_operator** out);
+void operator_free(fem_operator* op);
+int operator_apply_device(fem_operator* op, int k, const void* x, void* y);
+int operator_apply_host(fem_operator* op, int k, const double* x, double* y);
+int operator_time(fem_operator* op, int k, const void* x, void* y, int reps, double* ms);
+int operator_autotune(fem_operator* op, int k, const void* x, void* y, int reps, int* lanes, double* ms);
+int operator_set_lanes(fem_operator* op, int k, int lanes);
 }  // namespace femhip

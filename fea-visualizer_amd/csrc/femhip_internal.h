@@ -87,6 +87,9 @@ struct Inst {
   int* status;           // outcome bits
   long long* counters;   // [0] iterations, [1] sweeps, [2] steps
   double* last_res;      // [k] final relative residual of the last solve
+  double* traj;          // optional: primary variable after every traj_every-th
+  int traj_every;        //   step of a launch, at traj[(step + 1) / traj_every - 1][n][k]
+  int pad2_;
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

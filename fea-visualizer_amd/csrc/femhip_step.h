@@ -31,6 +31,10 @@ struct InstHost {
   int kv = 1;                // value columns
   long long formed_version = -1;
   bool dirty = true;         // matrices need (re)forming
+  int transpose = 0;         // form the transposed system matrix (adjoint solves)
+  int eq_override = -1;      // >= 0: equation id the next launches run (EQ_SOLVE)
+  double* traj = nullptr;    // trajectory record (see Inst::traj)
+  int traj_every = 0;
   double* d_S64 = nullptr;
   void* d_Si = nullptr;
   double* d_dinv64 = nullptr;

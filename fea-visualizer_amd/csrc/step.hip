@@ -861,18 +861,21 @@ __device__ void phase_bi_s_c(const Inst& I, int row_lo, int row_hi, const TI* r,
 // State update and the instability rule (CPUSolver.cpp:41-48, widened to
 // flag NaN/Inf separately).
 template <int K, typename TI, bool C>
-__device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags) {
+__device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags, int step) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   TI* xi = (TI*)I.xi;
   int f = 0;
+  double* traj = nullptr;  // trajectory record of this step, if any
+  if (I.traj && (step + 1) % I.traj_every == 0)
+    traj = I.traj + (size_t)((step + 1) / I.traj_every - 1) * I.n * K;
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
     double xf = ld<C>(I.x + o) + (double)ld<C>(xi + o);
     if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
     st<C>(I.x + o, xf);
     st<C>(xi + o, (TI)0);
-    double c1 = xf, c2 = 0.0;
+    double c1 = xf, c2 = 0.0, up = xf;  // up: new primary variable
     switch (I.eq) {
       case EQ_HEAT:
       case EQ_ADVDIFF:
@@ -884,6 +887,7 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
         const double un = ld<C>(I.u + o) + xf * I.dt;  // CPUSolver.cpp:118
         st<C>(I.u + o, un);
         c2 = un;
+        up = un;
       } break;
       case EQ_NEWMARK: {
         const double dt = I.dt, be = I.p2, ga = I.p3;
@@ -896,10 +900,12 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
         st<C>(I.a + o, xf);
         c1 = un;
         c2 = vn;
+        up = un;
       } break;
       default:
         break;
     }
+    if (traj) gp(traj)[o] = up;
     if (c1 > 1e4 || c2 > 1e4) f |= ST_UNSTABLE;
     if (!isfinite(c1) || !isfinite(c2)) f |= ST_NONFINITE;
   }
@@ -1167,7 +1173,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     int notconv = 0;
     for (int j = 0; j < K; j++) notconv |= sh.active[j];
     if (notconv) status |= ST_NOT_CONVERGED;
-    phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status);
+    phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status, step);
     if (rank == 0 && threadIdx.x < K) {
       const double bb = sh.bb[threadIdx.x];
       I.last_res[threadIdx.x] = bb > 0.0 ? sqrt(sh.ithr[threadIdx.x] / bb) : 0.0;

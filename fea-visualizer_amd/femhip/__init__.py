@@ -7,12 +7,12 @@ from . import _lib
 from ._lib import FemError
 from .meshes import (boundary_from_edges, icosphere, jittered_square, mesh_stats, read_obj,
                      structured_square)
-from .solver import (AdvectionDiffusionParameters, Batch, BoundaryCondition, Equation, FEMContext,
-                     HeatParameters, HIPSolver, NewmarkParameters, ReactionDiffusionParameters,
+from .solver import (Adjoint, AdvectionDiffusionParameters, Batch, BoundaryCondition, Equation, FEMContext,
+                     HeatParameters, HIPSolver, NewmarkParameters, Operator, ReactionDiffusionParameters,
                      SolverOptions, WaveParameters)
 
 __all__ = [
-    "FemError", "FEMContext", "HIPSolver", "Batch", "Equation", "BoundaryCondition", "SolverOptions",
+    "FemError", "FEMContext", "HIPSolver", "Batch", "Operator", "Adjoint", "Equation", "BoundaryCondition", "SolverOptions",
     "HeatParameters", "AdvectionDiffusionParameters", "WaveParameters", "ReactionDiffusionParameters",
     "NewmarkParameters", "structured_square", "jittered_square", "icosphere", "read_obj",
     "boundary_from_edges", "mesh_stats",

@@ -49,6 +49,9 @@ FEM_KRYLOV_BICGSTAB = 2
 FEM_KRYLOV_CG_CLASSIC = 3
 FEM_REACTION_NODAL = 0
 FEM_REACTION_LUMPED = 1
+FEM_SPMV_STREAM = 32
+FEM_SPMV_STREAM_NT = 33
+SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)
 
 
 class fem_mesh(C.Structure):
@@ -174,6 +177,19 @@ SIGNATURES = [
     ("fem_batch_advance", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_sync", C.c_int, [P]),
     ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),
+    ("fem_operator_create", C.c_int, [P, PD, I32, I32, C.POINTER(P)]),
+    ("fem_operator_destroy", C.c_int, [P]),
+    ("fem_operator_apply_device", C.c_int, [P, I32, P, P]),
+    ("fem_operator_apply", C.c_int, [P, I32, PD, PD]),
+    ("fem_operator_time", C.c_int, [P, I32, P, P, I32, PD]),
+    ("fem_operator_autotune", C.c_int, [P, I32, P, P, I32, PI, PD]),
+    ("fem_operator_set_schedule", C.c_int, [P, I32, I32]),
+    ("fem_operator_sync", C.c_int, [P]),
+    ("fem_adjoint_create", C.c_int, [P, C.POINTER(fem_solver_opts), C.POINTER(P)]),
+    ("fem_adjoint_destroy", C.c_int, [P]),
+    ("fem_adjoint_set_equation", C.c_int, [P, I32, P, C.c_size_t]),
+    ("fem_adjoint_forward", C.c_int, [P, PD, I32, I32, PD, C.POINTER(fem_step_stats)]),
+    ("fem_adjoint_backward", C.c_int, [P, PD, PD, PD, C.POINTER(fem_step_stats)]),
 ]
 
 _lib = None

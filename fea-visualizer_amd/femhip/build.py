@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "libfemhip.so")
 
-SOURCES = ["api.hip", "scan.hip", "ctx.hip", "batch.hip", "step.hip", "plain.hip"]
+SOURCES = ["api.hip", "scan.hip", "ctx.hip", "batch.hip", "step.hip", "plain.hip", "adjoint.hip", "operator.hip"]
 
 # -packed-fp32-ops: keep v_pk_*_f32 out of every code object.
 HIPCC_FLAGS = [

@@ -21,7 +21,7 @@ from . import _lib as L
 __all__ = [
     "HeatParameters", "AdvectionDiffusionParameters", "WaveParameters",
     "ReactionDiffusionParameters", "NewmarkParameters", "SolverOptions",
-    "FEMContext", "HIPSolver", "Batch", "Equation", "BoundaryCondition",
+    "FEMContext", "HIPSolver", "Batch", "Equation", "BoundaryCondition", "Operator", "Adjoint",
 ]
 
 
@@ -349,6 +349,120 @@ class Batch:
     def close(self):
         if getattr(self, "handle", None):
             self._lib.fem_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    """S = cM M + cK K + cA A of an assembled context (or its transpose) as a
+    stand-alone CSR operator: the sparse product of the reference's solvers
+    (CPUSolver.cpp:78,96,114,134,137; cgm.glsl:75-102)."""
+
+    def __init__(self, fem_ctx: FEMContext, coef=(0.0, 1.0, 0.0), precision: int = L.FEM_PREC_FP64,
+                 transpose: bool = False):
+        self._lib = L.load()
+        self.fem_ctx = fem_ctx
+        self.precision = precision
+        self.n = fem_ctx.num_unknowns()
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        h = C.c_void_p()
+        L.check(self._lib.fem_operator_create(fem_ctx.handle, L.ptr(c, C.c_double), int(precision),
+                                              1 if transpose else 0, C.byref(h)))
+        self.handle = h
+
+    def apply(self, x):
+        """y = S x for x of shape [N] or [N, nrhs] (host arrays, FP64 in and out)."""
+        xa = np.ascontiguousarray(x, dtype=np.float64)
+        k = 1 if xa.ndim == 1 else xa.shape[1]
+        y = np.empty_like(xa)
+        L.check(self._lib.fem_operator_apply(self.handle, k, L.ptr(xa, C.c_double), L.ptr(y, C.c_double)))
+        return y
+
+    def apply_device(self, nrhs: int, x_ptr: int, y_ptr: int):
+        """Enqueue y = S x on device pointers of the operator's value type."""
+        L.check(self._lib.fem_operator_apply_device(self.handle, int(nrhs), C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def time(self, nrhs: int, x_ptr: int, y_ptr: int, reps: int = 20) -> float:
+        """Average ms per application (HIP events on the operator's stream)."""
+        ms = C.c_double()
+        L.check(self._lib.fem_operator_time(self.handle, int(nrhs), C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                            int(reps), C.byref(ms)))
+        return ms.value
+
+    def autotune(self, nrhs: int, x_ptr: int, y_ptr: int, reps: int = 10):
+        """Pick the SpMV schedule for nrhs; returns (schedule, ms)."""
+        g = C.c_int32()
+        ms = C.c_double()
+        L.check(self._lib.fem_operator_autotune(self.handle, int(nrhs), C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                int(reps), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
+    def set_schedule(self, nrhs: int, schedule: int):
+        L.check(self._lib.fem_operator_set_schedule(self.handle, int(nrhs), int(schedule)))
+
+    def sync(self):
+        L.check(self._lib.fem_operator_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fem_operator_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Backward-Euler heat / advection-diffusion rollout with its discrete
+    adjoint: gradients of L(u_T) w.r.t. u_0 and the per-element diffusivity,
+    by checkpoint-and-recompute on the GPU."""
+
+    def __init__(self, fem_ctx: FEMContext, params, options: Optional[SolverOptions] = None, **kw):
+        self._lib = L.load()
+        self.fem_ctx = fem_ctx
+        self.options = options or SolverOptions(**kw)
+        h = C.c_void_p()
+        o = self.options.to_c()
+        L.check(self._lib.fem_adjoint_create(fem_ctx.handle, C.byref(o), C.byref(h)))
+        self.handle = h
+        self.forward_stats = L.fem_step_stats()
+        self.backward_stats = L.fem_step_stats()
+        self.set_equation(params)
+
+    def set_equation(self, params):
+        eq, cp = params.to_c()
+        L.check(self._lib.fem_adjoint_set_equation(self.handle, eq, C.byref(cp), C.sizeof(cp)))
+        self.params = params
+
+    def forward(self, u0, nsteps: int, checkpoint_every: int = 10):
+        """u_T of the rollout from u_0 (unknown space); keeps checkpoints."""
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        uT = np.empty_like(u)
+        L.check(self._lib.fem_adjoint_forward(self.handle, L.ptr(u, C.c_double), int(nsteps),
+                                              int(checkpoint_every), L.ptr(uT, C.c_double),
+                                              C.byref(self.forward_stats)))
+        return uT
+
+    def backward(self, grad_uT):
+        """(dL/du_0, dL/dkappa_e) for dL/du_T of the last forward."""
+        g = np.ascontiguousarray(grad_uT, dtype=np.float64)
+        gu = np.empty_like(g)
+        gk = np.empty(self.fem_ctx.mesh["tri"].shape[0], dtype=np.float64)
+        L.check(self._lib.fem_adjoint_backward(self.handle, L.ptr(g, C.c_double), L.ptr(gu, C.c_double),
+                                               L.ptr(gk, C.c_double), C.byref(self.backward_stats)))
+        return gu, gk
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fem_adjoint_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

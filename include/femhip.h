@@ -136,6 +136,7 @@ typedef struct fem_ctx fem_ctx;
 typedef struct fem_solver fem_solver;
 typedef struct fem_batch fem_batch;
 typedef struct fem_adjoint fem_adjoint;
+typedef struct fem_operator fem_operator;
 
 const char* fem_last_error(void);
 int fem_abi_version(void);
@@ -196,6 +197,56 @@ int fem_batch_sync(fem_batch* b);
 /* Autotuner over the number of workgroups the batch plan spreads over
  * (*chosen); state restored as for fem_solver_autotune. */
 int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, double* best_ms);
+
+/* ---- stand-alone system operator: S = coef[0] M + coef[1] K + coef[2] A of
+ * an assembled context (or its transpose), stored once in CSR with FP32
+ * (FEM_PREC_FP32_IR) or FP64 (FEM_PREC_FP64) values.  Replaces the sparse
+ * products of the reference's solvers: Eigen's S * x at
+ * src/FEM/CPUSolver.cpp:78,96,114,134,137 and the ELL row loop Ad_i of
+ * shaders/FEM/cgm.glsl:75-102.  Columns: nrhs in {1, 2, 4, 16}, vectors
+ * stored [N][nrhs]. ---- */
+int fem_operator_create(fem_ctx* ctx, const double coef[3], int32_t precision, int32_t transpose,
+                        fem_operator** out);
+int fem_operator_destroy(fem_operator* op);
+/* y = S x on device pointers of the operator's value type (float or double),
+ * enqueued on the operator's stream (returns before completion). */
+int fem_operator_apply_device(fem_operator* op, int32_t nrhs, const void* x, void* y);
+/* y = S x on host arrays of doubles (converted to the value type), synchronous. */
+int fem_operator_apply(fem_operator* op, int32_t nrhs, const double* x, double* y);
+/* Average device time of `reps` back-to-back applications, HIP events on
+ * the operator's stream. */
+int fem_operator_time(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps, double* ms);
+/* SpMV schedules: 1, 2, 4, 8 or 16 lanes per row (>= nrhs), or the row-block
+ * stream (a workgroup stages its rows' column/value stream in LDS with 16-byte
+ * loads), optionally with non-temporal stream loads. */
+enum { FEM_SPMV_STREAM = 32, FEM_SPMV_STREAM_NT = 33 };
+/* Schedule autotuner: times every schedule valid for nrhs and keeps the
+ * fastest (*schedule receives it). */
+int fem_operator_autotune(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps,
+                          int32_t* schedule, double* ms);
+/* Fix the schedule used for nrhs columns. */
+int fem_operator_set_schedule(fem_operator* op, int32_t nrhs, int32_t schedule);
+int fem_operator_sync(fem_operator* op);
+
+/* ---- adjoint of a backward-Euler rollout (no reference counterpart; the
+ * forward is the heat / advection-diffusion step of
+ * src/FEM/CPUSolver.cpp:72-102 with per-element diffusivity).  The forward
+ * keeps every `checkpoint_every`-th state; the backward recomputes each
+ * segment from its checkpoint and runs transposed solves.  Gradients are
+ * returned for a loss L(u_T) given dL/du_T: dL/du_0 (unknown space) and
+ * dL/dkappa_e (one per triangle, for K = sum_e kappa_e K_e as assembled with
+ * fem_assembly_params.kappa_e). ---- */
+int fem_adjoint_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_adjoint** out);
+int fem_adjoint_destroy(fem_adjoint* a);
+/* FEM_HEAT or FEM_ADVECTION_DIFFUSION with their parameter structs. */
+int fem_adjoint_set_equation(fem_adjoint* a, int32_t equation, const void* params, size_t size);
+/* u0 and uT: [N] unknowns.  stats->ms is device time of the rollout. */
+int fem_adjoint_forward(fem_adjoint* a, const double* u0, int32_t nsteps, int32_t checkpoint_every, double* uT,
+                        fem_step_stats* stats);
+/* grad_uT: dL/du_T [N]; grad_u0 [N] and grad_kappa_e [num_triangles] may be
+ * NULL.  stats->ms is the wall time of the whole backward pass. */
+int fem_adjoint_backward(fem_adjoint* a, const double* grad_uT, double* grad_u0, double* grad_kappa_e,
+                         fem_step_stats* stats);
 
 #ifdef __cplusplus
 }
