@@ -52,12 +52,12 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
-def step_bytes(n, nnz, k, kv, iters, sweeps, steps):
+def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4):
     """Algorithmic HBM bytes of the persistent stepping kernel (int32 indices,
-    FP32 inner vectors and values, FP64 refinement), every vector element and
-    matrix entry touched once per phase (cache-perfect gathers)."""
-    it = nnz * (4 + 4 * kv) + n * (11 * k * 4 + 4 * kv)       # CG phases A + B
-    sw = nnz * (4 + 8 * kv) + n * (28 * k + 4 * kv)           # FP64 residual
+    inner vectors and values of ti bytes, FP64 refinement), every vector
+    element and matrix entry touched once per phase (cache-perfect gathers)."""
+    it = nnz * (4 + ti * kv) + n * (11 * k * ti + ti * kv)     # one Krylov iteration
+    sw = nnz * (4 + 8 * kv) + n * (3 * k * 8 + k * ti + ti * kv)  # FP64 residual
     st = nnz * (12 + 8 * kv) + n * (36 * k + 4 * kv) + n * 32 * k  # RHS + update
     return iters * it + sweeps * sw + steps * st
 
@@ -205,10 +205,274 @@ def adjoint_measure(side=512, nsteps=100, every=10, fd_dirs=3):
     return out
 
 
+def contract_line(metric, value, unit, steps, warmup, ms_per_step, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def gauss_sum(X, centres, sigmas):
+    return np.exp(-((X[:, None, 0] - centres[None, :, 0]) ** 2 + (X[:, None, 1] - centres[None, :, 1]) ** 2)
+                  / (2 * np.asarray(sigmas)[None, :] ** 2))
+
+
+def run_heat_c1(args):
+    """Heat equation on the structured 64 x 64 unit square (8,192 triangles,
+    4,225 vertices, 3,969 unknowns), backward Euler dt = 1e-3, conductivity
+    1, zero Dirichlet boundary, u0 = exp(-|x - (1/2,1/2)|^2 / 0.01), 100
+    steps, FP64 CG to tol 1e-10 (the persistent kernel in FP64)."""
+    from femhip import Equation, FEMContext, HeatParameters, HIPSolver, SolverOptions, _lib, meshes
+    steps = args.steps or 100
+    mesh = meshes.structured_square(64)
+    ctx = FEMContext(mesh, 0)
+    n, nnz = ctx.num_unknowns(), ctx.info().nnz
+    X = mesh["xyz"].astype(np.float64)
+    u0 = np.exp(-((X[:, 0] - 0.5) ** 2 + (X[:, 1] - 0.5) ** 2) / 0.01)[ctx.idx_map >= 0]
+    ctx.equation = Equation.Heat
+    ctx.parameters[Equation.Heat] = HeatParameters(1e-3, 1.0)
+    sol = HIPSolver(ctx, SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-10))
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    tuned = sol.autotune(20) if not args.no_autotune else None
+    sol.advance(args.warmup)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    sol.sync()
+    t = time.perf_counter()
+    status = sol.advance(steps)
+    sol.sync()
+    wall = time.perf_counter() - t
+    st = sol.last_stats
+    ach = step_bytes(n, nnz, 1, 1, st.iterations, st.sweeps, steps, ti=8) / (st.ms * 1e-3) / 1e9
+    ref = HIPSolver(ctx, SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-13))
+    ref.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    ref.advance(steps)
+    uu, ur = sol.get_state_unknowns(), ref.get_state_unknowns()
+    out = contract_line(METRIC, n * steps / wall, "DOF·steps/s", steps, args.warmup, 1e3 * wall / steps, "f64",
+                        {"workload": f"heat, structured 64x64 unit square (N={n}, nnz={nnz}), dt=1e-3, "
+                                     f"conductivity 1, Dirichlet, Gaussian u0, FP64 CG tol 1e-10",
+                         "global_batch": 1, "parallelism": "single GPU, persistent cooperative kernel"},
+                        status=int(status),
+                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
+                                "kernel_ms": st.ms},
+                        roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step<1,double>",
+                                  "note": "1.6 MB working set: cache-resident, latency-bound"},
+                        accuracy={"trajectory_rel_l2_final": float(np.linalg.norm(uu - ur) / np.linalg.norm(ur)),
+                                  "reference": "FP64 GPU CG, tol 1e-13"})
+    if not args.no_cpu:
+        import oracle
+        s = oracle.System(mesh, bc=0, dtype=np.float32)
+        prep = oracle.port_prepare(s)
+        t = time.perf_counter()
+        its, _, _ = oracle.port_run(prep, oracle.HEAT, [1e-3, 1.0], steps, u0)
+        dt_ = time.perf_counter() - t
+        model, ncpu = cpu_info()
+        out["cpu_baseline"] = {"value": n * steps / dt_, "unit": "DOF·steps/s", "cores": 1, "kind": "port",
+                               "sample": f"all {steps} steps, FP32 Eigen-default Jacobi-CG port ({its} CG "
+                                         f"iterations), gcc -O3 -DNDEBUG; host: {model} ({ncpu} CPUs)"}
+    return out
+
+
+def run_wave_c3(args):
+    """Wave equation on the jittered 1024 x 1024 lattice (interior vertices
+    moved by U(+-h/4), seed 2, Delaunay split; 1,050,625 vertices, 2,097,152
+    triangles), c = 1, Newmark beta = 1/4, gamma = 1/2, dt = 2e-4, u0 = sum of
+    8 Gaussian bumps (sigma 0.02, centres U[0.1,0.9]^2, seed 3), v0 = 0, a0
+    consistent (M a0 = -c^2 K u0), 500 steps, FP32 CG + FP64 refinement."""
+    from femhip import Equation, FEMContext, HIPSolver, NewmarkParameters, SolverOptions, WaveParameters, _lib, meshes
+    steps = args.steps or 500
+    mesh = meshes.jittered_square(1024, seed=2)
+    ctx = FEMContext(mesh, 0)
+    n, nnz = ctx.num_unknowns(), ctx.info().nnz
+    rng = np.random.Generator(np.random.PCG64(3))
+    cen = rng.uniform(0.1, 0.9, size=(8, 2))
+    X = mesh["xyz"].astype(np.float64)[ctx.idx_map >= 0]
+    u0 = gauss_sum(X, cen, [0.02] * 8).sum(axis=1)
+    ctx.equation = Equation.Wave_Newmark
+    ctx.parameters[Equation.Wave_Newmark] = NewmarkParameters(2e-4, 1.0, 0.25, 0.5)
+
+    def fresh(**kw):
+        sol = HIPSolver(ctx, SolverOptions(**kw))
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+        sol.init_newmark_acceleration()
+        return sol
+    sol = fresh(tol=1e-10)
+    a0 = sol.get_state_unknowns(_lib.FEM_FIELD_A)
+    tuned = sol.autotune(5) if not args.no_autotune else None
+    sol.advance(args.warmup)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, np.zeros(n))
+    sol.set_state_unknowns(_lib.FEM_FIELD_A, a0)
+    sol.sync()
+    t = time.perf_counter()
+    status = sol.advance(steps)
+    sol.sync()
+    wall = time.perf_counter() - t
+    st = sol.last_stats
+    ach = step_bytes(n, nnz, 1, 1, st.iterations, st.sweeps, steps) / (st.ms * 1e-3) / 1e9
+    ref = fresh(precision=_lib.FEM_PREC_FP64, tol=1e-13)
+    ref.advance(steps)
+    uu, ur = sol.get_state_unknowns(), ref.get_state_unknowns()
+    out = contract_line(METRIC, n * steps / wall, "DOF·steps/s", steps, args.warmup, 1e3 * wall / steps,
+                        "f32+f64 (FP32 CG, FP64 refinement residual and dots)",
+                        {"workload": f"Newmark wave, jittered 1024x1024 lattice (N={n}, nnz={nnz}), c=1, "
+                                     f"beta=1/4 gamma=1/2, dt=2e-4, 8 Gaussian bumps, tol 1e-10",
+                         "global_batch": 1, "parallelism": "single GPU, persistent cooperative kernel"},
+                        status=int(status),
+                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
+                                "kernel_ms": st.ms, "iters_per_step": st.iterations / steps},
+                        roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step<1,float>",
+                                  "note": "~200 MB working set: mostly Infinity-Cache resident"},
+                        accuracy={"trajectory_rel_l2_final": float(np.linalg.norm(uu - ur) / np.linalg.norm(ur)),
+                                  "reference": "FP64 GPU CG, tol 1e-13"})
+    # the reference's own wave scheme (CPUSolver.cpp:108-121) on the same mesh
+    ctx.equation = Equation.Wave
+    ctx.parameters[Equation.Wave] = WaveParameters(2e-4, 1.0)
+    w = HIPSolver(ctx, SolverOptions(tol=1e-10))
+    w.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    w.advance(2)
+    w.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    w.set_state_unknowns(_lib.FEM_FIELD_V, np.zeros(n))
+    w.sync()
+    t = time.perf_counter()
+    w.advance(steps)
+    w.sync()
+    out["reference_wave_scheme"] = {"value": n * steps / (time.perf_counter() - t), "unit": "DOF·steps/s",
+                                    "iterations": int(w.last_stats.iterations)}
+    if not args.no_cpu:
+        import oracle
+        s = oracle.System(mesh, bc=0, dtype=np.float32)
+        prep = oracle.port_prepare(s)
+        ns = 3
+        t = time.perf_counter()
+        its, _, _ = oracle.port_run(prep, oracle.WAVE, [2e-4, 1.0], ns, u0, np.zeros(n))
+        dt_ = time.perf_counter() - t
+        model, ncpu = cpu_info()
+        out["cpu_baseline"] = {"value": n * ns / dt_, "unit": "DOF·steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{ns} steps of the reference's wave scheme (CPUSolver.cpp:108-121; "
+                                         f"Newmark has no reference counterpart) on the same mesh, FP32 "
+                                         f"Eigen-default CG port ({its} iterations); host: {model} ({ncpu} CPUs)"}
+    return out
+
+
+def c4_instances(ninst=256):
+    """The 256 batched instances: side = round(exp(U(ln 32, ln 512))),
+    kappa = exp(U(ln 1e-4, ln 1e-1)), v = U(0,1) (cos t, sin t, 0) (seed 4);
+    jittered Delaunay-split lattices (seed 1000 + i); 16 Gaussian bumps per
+    mesh, centre U[0.2,0.8]^2 and sigma U(0.02, 0.1) (seed 5)."""
+    from femhip import meshes
+    r4 = np.random.Generator(np.random.PCG64(4))
+    r5 = np.random.Generator(np.random.PCG64(5))
+    out = []
+    for i in range(ninst):
+        side = int(round(np.exp(r4.uniform(np.log(32), np.log(512)))))
+        kap = float(np.exp(r4.uniform(np.log(1e-4), np.log(1e-1))))
+        sp, th = r4.uniform(0, 1), r4.uniform(0, 2 * np.pi)
+        cen = r5.uniform(0.2, 0.8, size=(16, 2))
+        sig = r5.uniform(0.02, 0.1, size=16)
+        out.append((meshes.jittered_square(side, seed=1000 + i), kap, (sp * np.cos(th), sp * np.sin(th), 0.0),
+                    cen, sig))
+    return out
+
+
+def run_batched_c4(args):
+    """256 advection-diffusion instances of uneven size (about 1.3e7
+    unknowns in all), 16 right-hand sides each, backward Euler dt = 1e-3,
+    unnormalised constant velocity per instance; one persistent launch for
+    all instances and steps, FP32 BiCGSTAB + FP64 refinement (tol 1e-10)."""
+    from femhip import AdvectionDiffusionParameters, Batch, FEMContext, SolverOptions, _lib
+    steps = args.steps or 200
+    inst = c4_instances(args.instances)
+    ctxs, fields, prms = [], [], []
+    tot_n = tot_nnz = 0
+    for m, kap, vel, cen, sig in inst:
+        c = FEMContext(m, 0, velocity=vel, normalize_velocity=False)
+        tot_n += c.num_unknowns()
+        tot_nnz += c.info().nnz
+        X = m["xyz"].astype(np.float64)[c.idx_map >= 0]
+        ctxs.append(c)
+        fields.append(np.ascontiguousarray(gauss_sum(X, cen, sig)))
+        prms.append(AdvectionDiffusionParameters(1e-3, kap))
+
+    def make(**kw):
+        b = Batch(ctxs, SolverOptions(nrhs=16, **kw))
+        for i in range(len(ctxs)):
+            b.set_equation(i, prms[i])
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
+        return b
+    b = make(tol=1e-10)
+    tuned = b.autotune(2) if not args.no_autotune else None
+    for i in range(len(ctxs)):
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
+    b.sync()
+    t = time.perf_counter()
+    status = b.advance(steps)
+    b.sync()
+    wall = time.perf_counter() - t
+    st = b.last_stats
+    dof = 16 * tot_n
+    # iterations and sweeps are summed over instances: apply their mean to the totals
+    ni = max(1, len(ctxs))
+    ach = step_bytes(tot_n, tot_nnz, 16, 1, st.iterations / ni, st.sweeps / ni, steps) / (st.ms * 1e-3) / 1e9
+    finite = all(np.all(np.isfinite(b.get_state_unknowns(i))) for i in range(len(ctxs)))
+    out = contract_line(METRIC, dof * steps / wall, "DOF·steps/s", steps, 0, 1e3 * wall / steps,
+                        "f32+f64 (FP32 BiCGSTAB, FP64 refinement residual and dots)",
+                        {"workload": f"batched advection-diffusion, {len(ctxs)} jittered meshes of side "
+                                     f"log-U[32,512] (sum N={tot_n}, sum nnz={tot_nnz}), 16 RHS each, dt=1e-3, "
+                                     f"tol 1e-10", "global_batch": len(ctxs) * 16,
+                         "parallelism": "single GPU, one persistent launch over all instances"},
+                        status=int(status), all_finite=bool(finite),
+                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
+                                "kernel_ms": st.ms},
+                        roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step<16,float,BiCGSTAB>"})
+    if not args.no_accuracy:
+        ref = make(precision=_lib.FEM_PREC_FP64, tol=1e-13)
+        ref.advance(steps)
+        errs = []
+        for i in range(len(ctxs)):
+            a, r = b.get_state_unknowns(i), ref.get_state_unknowns(i)
+            errs.append(float(np.linalg.norm(a - r) / np.linalg.norm(r)))
+        out["accuracy"] = {"rel_l2_max_over_instances": float(np.max(errs)), "reference": "FP64 GPU, tol 1e-13"}
+        ref.close()
+    b.close()
+    if not args.no_plain:
+        pl = make(engine=_lib.FEM_ENGINE_PLAIN, precision=_lib.FEM_PREC_FP64)
+        pl.sync()
+        t = time.perf_counter()
+        pl.advance(1)
+        pl.sync()
+        pv = dof / (time.perf_counter() - t)
+        out["plain_fp64_gpu"] = {"value": pv, "unit": "DOF·steps/s", "steps": 1, "speedup_of_value": out["value"] / pv}
+        pl.close()
+    if not args.no_cpu:
+        import oracle
+        m, kap, vel, cen, sig = inst[0]
+        s = oracle.System(m, bc=0, velocity=vel, normalize=False, dtype=np.float32)
+        prep = oracle.port_prepare(s)
+        t = time.perf_counter()
+        for j in range(16):
+            oracle.port_run(prep, oracle.ADVDIFF, [1e-3, kap], 2, fields[0][:, j])
+        dt_ = time.perf_counter() - t
+        model, ncpu = cpu_info()
+        out["cpu_baseline"] = {"value": 16 * s.n * 2 / dt_, "unit": "DOF·steps/s", "cores": 1, "kind": "port",
+                               "sample": f"instance 0 (N={s.n}), 16 columns x 2 steps, FP32 Eigen-default CG port "
+                                         f"(the reference solves advection-diffusion with CG); host: {model} "
+                                         f"({ncpu} CPUs)"}
+    for c in ctxs:
+        c.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--workload", default="grayscott", choices=["grayscott", "heat", "wave", "batched"],
+                    help="grayscott: the headline line; heat / wave / batched: the other workloads")
+    ap.add_argument("--instances", type=int, default=256, help="batched: number of meshes")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--level", type=int, default=6)
     ap.add_argument("--cpu-steps", type=int, default=2000)
@@ -238,6 +502,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload != "grayscott":
+        fn = {"heat": run_heat_c1, "wave": run_wave_c3, "batched": run_batched_c4}[args.workload]
+        print(json.dumps(fn(args)))
+        return
+    if args.steps is None:
+        args.steps = 2000
 
     from femhip import Equation, FEMContext, HIPSolver, ReactionDiffusionParameters, SolverOptions, _lib, meshes
 

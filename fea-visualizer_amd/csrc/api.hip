@@ -402,6 +402,21 @@ int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, doubl
   })
 }
 
+int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
+                  fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!ctx || !coef || (ctx->n > 0 && (!b || !x))) {
+      set_error("fem_ctx_solve: null argument");
+      return FEM_ERR_INVALID;
+    }
+    if (opts && opts->engine != FEM_ENGINE_PERSISTENT) {
+      set_error("fem_ctx_solve runs on the persistent engine");
+      return FEM_ERR_INVALID;
+    }
+    return linear_solve(ctx, coef, opts, b, x, stats);
+  })
+}
+
 // ----------------------------------------------------------------- operator
 
 int fem_operator_create(fem_ctx* ctx, const double coef[3], int32_t precision, int32_t transpose,

@@ -166,6 +166,11 @@ int form_inst(fem_batch* b, InstHost& h) {
       cf.cM[0] = 1.0;
       cf.cK[0] = h.p2 * dt * dt * h.p1 * h.p1;
       break;
+    case EQ_SOLVE:  // one linear solve with caller-given coefficients
+      cf.cM[0] = h.p1;
+      cf.cK[0] = h.p2;
+      cf.cA[0] = h.p3;
+      break;
     default:
       set_error("form: equation not set");
       return FEM_ERR_STATE;
@@ -198,7 +203,9 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.kv = h.kv;
   I.eq = (h.eq_override >= 0) ? h.eq_override : h.eq;
   int kr = o.krylov;
-  if (kr == FEM_KRYLOV_AUTO) kr = (h.eq == FEM_ADVECTION_DIFFUSION) ? FEM_KRYLOV_BICGSTAB : FEM_KRYLOV_CG;
+  if (kr == FEM_KRYLOV_AUTO)
+    kr = (h.eq == FEM_ADVECTION_DIFFUSION || (h.eq == EQ_SOLVE && h.p3 != 0.0)) ? FEM_KRYLOV_BICGSTAB
+                                                                              : FEM_KRYLOV_CG;
   I.krylov = (kr == FEM_KRYLOV_BICGSTAB) ? 1 : (kr == FEM_KRYLOV_CG_CLASSIC) ? 2 : 0;
   I.warm = o.warm_start;
   I.max_inner = o.max_inner > 0 ? o.max_inner : 200;
@@ -390,8 +397,20 @@ int build_plan(fem_batch* b) {
   b->launch.k = k;
   b->launch.inner_fp32 = inner_fp32;
   b->launch.kr = kr;
-  b->launch.coh = 0;
-  for (int g : sizes) b->launch.coh |= (g > 1) ? 1 : 0;
+  // exchange mode: a workgroup of a multi-workgroup team whose rows exceed
+  // two per lane (the LDS row cache) streams its phases: fenced barriers
+  bool multi = false, streaming = false;
+  for (int t = 0; t < nteams; t++) {
+    if (sizes[t] <= 1) continue;
+    multi = true;
+    for (int i : teams[t]) {
+      const long long rows = (b->insts[i].ctx->n + sizes[t] - 1) / sizes[t];
+      if (rows * k > 2LL * kBlock) streaming = true;
+    }
+  }
+  b->launch.coh = !multi ? 0
+                  : b->mode_override ? b->mode_override
+                  : streaming ? kExchangeFenced : kExchangeCoherent;
   b->nteams = nteams;
   b->grid = grid;
   b->team_sizes = sizes;
@@ -534,6 +553,61 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
 bad_size:
   set_error("parameter struct size does not match the equation");
   return FEM_ERR_INVALID;
+}
+
+// Instance `inst` solves (coef[0] M + coef[1] K + coef[2] A) x = b once per
+// advance; b and the guess x are set by the caller.
+int batch_set_linear(fem_batch* b, int inst, const double coef[3]) {
+  if (inst < 0 || inst >= (int)b->insts.size()) {
+    set_error("bad instance");
+    return FEM_ERR_INVALID;
+  }
+  InstHost& h = b->insts[inst];
+  if (h.k != b->nrhs || h.kv != 1 || !h.d_u) {
+    const int rc = alloc_inst(b, h, b->nrhs, 1);
+    if (rc) return rc;
+    b->plan_ready = false;
+  }
+  if (h.eq != EQ_SOLVE) b->plan_ready = false;
+  h.eq = EQ_SOLVE;
+  h.dt = 1.0;
+  h.p1 = coef[0];
+  h.p2 = coef[1];
+  h.p3 = coef[2];
+  h.p4 = 0.0;
+  h.dirty = true;
+  return 0;
+}
+
+// One solve of (coef[0] M + coef[1] K + coef[2] A) x = b for nrhs columns
+// ([N][nrhs] host arrays); x is the initial guess when opts->warm_start.
+int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* bh, double* xh,
+                 fem_step_stats* stats) {
+  fem_batch* b = nullptr;
+  int rc = batch_create(&c, 1, opts, &b);
+  if (rc) return rc;
+  rc = batch_set_linear(b, 0, coef);
+  InstHost& h = b->insts[0];
+  const size_t bytes = sizeof(double) * (size_t)c->n * b->nrhs;
+  if (!rc && c->n > 0) {
+    hipError_t e = hipMemcpyAsync(h.d_b, bh, bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+      e = b->opts.warm_start ? hipMemcpyAsync(h.d_x, xh, bytes, hipMemcpyHostToDevice, b->stream)
+                             : hipMemsetAsync(h.d_x, 0, bytes, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "linear_solve: upload");
+  }
+  int out = rc;
+  if (!rc) {
+    out = batch_advance(b, 1, stats);
+    if (out >= 0 && c->n > 0) {
+      hipError_t e = hipMemcpyAsync(xh, h.d_x, bytes, hipMemcpyDeviceToHost, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) out = hip_fail(e, "linear_solve: download");
+    }
+  }
+  batch_free(b);
+  return out;
 }
 
 // Map a state field to (device array, first column, column count).
@@ -749,25 +823,30 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     }
     return 0;
   };
-  const int saved_team = b->opts.team_size, saved_budget = b->grid_budget;
+  const int saved_team = b->opts.team_size, saved_budget = b->grid_budget, saved_mode = b->mode_override;
   double best = 1e300;
-  int best_c = single ? saved_team : saved_budget;
+  int best_c = single ? saved_team : saved_budget, best_m = saved_mode;
   if (!rc) {
     for (int c : cands) {
-      if (single) b->opts.team_size = c;
-      else b->grid_budget = c;
-      b->plan_ready = false;
-      if ((rc = restore())) break;
-      fem_step_stats st{};
-      const int r = batch_advance(b, probe, &st);
-      if (r < 0) {
-        rc = r;
-        break;
+      for (int m : {kExchangeCoherent, kExchangeFenced}) {
+        if (single) b->opts.team_size = c;
+        else b->grid_budget = c;
+        b->mode_override = m;
+        b->plan_ready = false;
+        if ((rc = restore())) break;
+        fem_step_stats st{};
+        const int r = batch_advance(b, probe, &st);
+        if (r < 0) {
+          rc = r;
+          break;
+        }
+        if (st.ms < best) {
+          best = st.ms;
+          best_c = c;
+          best_m = m;
+        }
       }
-      if (st.ms < best) {
-        best = st.ms;
-        best_c = c;
-      }
+      if (rc) break;
     }
   }
   const int rr = restore();
@@ -776,6 +855,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   for (double* d : snap) (void)hipFree(d);
   if (single) b->opts.team_size = rc ? saved_team : best_c;
   else b->grid_budget = rc ? saved_budget : best_c;
+  b->mode_override = rc ? saved_mode : best_m;
   b->plan_ready = false;
   if (chosen) *chosen = best_c;
   if (best_ms) *best_ms = best;

@@ -15,8 +15,13 @@ struct StepLaunch {
   int k = 1;
   int inner_fp32 = 1;
   int kr = 0;   // Krylov method: 0 pipelined CG, 1 BiCGSTAB, 2 classic CG
-  int coh = 1;  // 1 when any team spans more than one workgroup
+  int coh = 1;  // exchange mode: 0 one-workgroup teams only, 1 coherent, 2 fenced
 };
+
+// Exchange mode for teams spanning several workgroups: coherent accesses
+// while a workgroup's rows fit the kernel's LDS row cache (latency-bound
+// phases), fenced barriers beyond (bandwidth-bound phases).
+constexpr int kExchangeCoherent = 1, kExchangeFenced = 2;
 
 int launch_step(const StepLaunch& L, int nsteps, hipStream_t st);
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
@@ -72,6 +77,7 @@ struct fem_batch {
   int* d_abort = nullptr;
   int nteams = 0;
   int grid_budget = 0;          // multi-instance plans: workgroups to spread over (0 = all co-resident)
+  int mode_override = 0;        // exchange mode forced by the autotuner (0 = by the row-cache rule)
   std::vector<int> team_sizes;  // for reporting
   // plain-engine scratch
   double* d_plain = nullptr;
@@ -93,4 +99,7 @@ int batch_clear(fem_batch* b, int inst);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
+int batch_set_linear(fem_batch* b, int inst, const double coef[3]);
+int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
+                 fem_step_stats* stats);
 }  // namespace femhip

@@ -156,6 +156,7 @@ SIGNATURES = [
     ("fem_ctx_get_idx_map", C.c_int, [P, PI]),
     ("fem_ctx_get_pattern", C.c_int, [P, PI, PI]),
     ("fem_ctx_get_matrix", C.c_int, [P, I32, PD]),
+    ("fem_ctx_solve", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
     ("fem_solver_create", C.c_int, [P, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_solver_destroy", C.c_int, [P]),
     ("fem_solver_set_equation", C.c_int, [P, I32, P, C.c_size_t]),

@@ -1,9 +1,17 @@
-"""Build the HIP library (libfemhip.so) in-tree for gfx950 with hipcc."""
+"""Build the HIP library (libfemhip.so) in-tree for gfx950 with hipcc.
+
+Every source compiles to its own object in parallel (the stepping kernel is
+split into one translation unit per exchange mode), then one link step makes
+the shared library.  Objects go to a temporary directory; only the library
+is written into the package.
+"""
 from __future__ import annotations
 
+import concurrent.futures as cf
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -12,29 +20,46 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "libfemhip.so")
 
-SOURCES = ["api.hip", "scan.hip", "ctx.hip", "batch.hip", "step.hip", "plain.hip", "adjoint.hip", "operator.hip"]
+SOURCES = ["api.hip", "scan.hip", "ctx.hip", "batch.hip", "step.hip", "step_plain.hip", "step_coherent.hip",
+           "step_fenced.hip", "plain.hip", "adjoint.hip", "operator.hip"]
 
 # -packed-fp32-ops: keep v_pk_*_f32 out of every code object.
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
 
-def hipcc_command(out: str = OUT):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    return [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", out + ".tmp",
-            *[os.path.join(CSRC, s) for s in SOURCES]]
+def _hipcc():
+    return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def build(out: str = OUT, verbose: bool = False) -> str:
-    cmd = hipcc_command(out)
+def compile_command(src: str, obj: str):
+    return [_hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, src), "-o", obj]
+
+
+def link_command(objs, out: str):
+    return [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out, *objs]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + res.stdout + res.stderr)
+
+
+def build(out: str = OUT, verbose: bool = False, jobs: int | None = None) -> str:
+    jobs = jobs or min(len(SOURCES), max(1, min(16, os.cpu_count() or 1)))
+    with tempfile.TemporaryDirectory(prefix="femhip_build_") as tmp:
+        objs = [os.path.join(tmp, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+        with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+            futs = [ex.submit(_run, compile_command(s, o), verbose) for s, o in zip(SOURCES, objs)]
+            for f in futs:
+                f.result()
+        _run(link_command(objs, out + ".tmp"), verbose)
     os.replace(out + ".tmp", out)
     return out
 

@@ -192,6 +192,23 @@ class FEMContext:
         L.check(self._lib.fem_ctx_get_matrix(self.handle, int(which), L.ptr(out, C.c_double)))
         return out[: i.nnz]
 
+    def solve(self, coef, b, options: Optional[SolverOptions] = None, x0=None):
+        """x with (coef[0] M + coef[1] K + coef[2] A) x = b (b: [N] or [N, nrhs]),
+        solved on the GPU with the given solver options."""
+        o = options or SolverOptions()
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        k = 1 if bb.ndim == 1 else bb.shape[1]
+        co = o.to_c()
+        co.nrhs = k
+        co.warm_start = 1 if x0 is not None else 0
+        x = np.zeros_like(bb) if x0 is None else np.array(x0, dtype=np.float64).reshape(bb.shape)
+        cf = np.ascontiguousarray(coef, dtype=np.float64)
+        st = L.fem_step_stats()
+        L.check(self._lib.fem_ctx_solve(self.handle, L.ptr(cf, C.c_double), C.byref(co), L.ptr(bb, C.c_double),
+                                        L.ptr(x, C.c_double), C.byref(st)))
+        self.last_solve_stats = st
+        return x
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.fem_ctx_destroy(self.handle)
@@ -280,6 +297,19 @@ class HIPSolver:
 
     def sync(self):
         L.check(self._lib.fem_solver_sync(self.handle))
+
+    def init_newmark_acceleration(self):
+        """Consistent initial acceleration of the Newmark wave from the current
+        u: M a = -c^2 K u (both products and the solve on the GPU)."""
+        prm = self.fem_ctx.parameters[Equation.Wave_Newmark]
+        u = self.get_state_unknowns(L.FEM_FIELD_U)
+        op = Operator(self.fem_ctx, (0.0, -prm.c * prm.c, 0.0), L.FEM_PREC_FP64)
+        rhs = op.apply(u)
+        op.close()
+        o = SolverOptions(precision=L.FEM_PREC_FP64, tol=min(self.options.tol, 1e-12))
+        a = self.fem_ctx.solve((1.0, 0.0, 0.0), rhs, o)
+        self.set_state_unknowns(L.FEM_FIELD_A, a)
+        return a
 
     def autotune(self, probe_steps: int = 20):
         """Pick the team size by timing probe_steps under a few launch plans

@@ -158,6 +158,13 @@ int fem_ctx_info_get(const fem_ctx* ctx, fem_ctx_info* info);
 int fem_ctx_get_idx_map(const fem_ctx* ctx, int32_t* idx_map);
 int fem_ctx_get_pattern(const fem_ctx* ctx, int32_t* row_ptr, int32_t* col_idx);
 int fem_ctx_get_matrix(const fem_ctx* ctx, int32_t which, double* values);
+/* One solve of (coef[0] M + coef[1] K + coef[2] A) x = b with the solver
+ * options (precision, Krylov method, tolerance; nrhs columns, arrays
+ * [N][nrhs]): the cg.compute / cg.solve pair of CPUSolver.cpp:80-81 for a
+ * caller-given matrix combination.  x is the initial guess when
+ * opts->warm_start, else x0 = 0 as Eigen's solve(). */
+int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
+                  fem_step_stats* stats);
 
 /* ---- solver: the Solver interface (Solver.hpp:8-15) ----
  * fem_solver_advance  <- Solver::advance_time        (CPUSolver.cpp:64-147)

@@ -184,3 +184,36 @@ def test_torch_autograd_rollout(gpu):
     assert rel(tu0.grad.numpy(), gu) <= 1e-9
     assert rel(tk.grad.numpy(), gk) <= 1e-9
     roll.close()
+
+
+@pytest.mark.parametrize("coef", [(1.0, 0.0, 0.0), (500.0, 1.0, -1.0)])
+def test_ctx_solve_vs_direct(gpu, coef):
+    """fem_ctx_solve against a direct FP64 solve of the oracle's matrices."""
+    import scipy.sparse.linalg as sla
+    m, ve, kappa = c5_like(28, 15)
+    ctx = FEMContext(m, 0, velocity_e=ve, normalize_velocity=False, kappa_e=kappa)
+    s = oracle.System(m, bc=0, velocity_e=ve, normalize=False, kappa_e=kappa)
+    S = (coef[0] * s.scipy("M") + coef[1] * s.scipy("K") + coef[2] * s.scipy("A")).tocsc()
+    rng = np.random.Generator(np.random.PCG64(16))
+    b = rng.standard_normal((s.n, 4))
+    ref = sla.spsolve(S, b)
+    x = ctx.solve(coef, b, SolverOptions(tol=1e-12))
+    assert rel(x, ref) <= 1e-10, rel(x, ref)
+    x1 = ctx.solve(coef, b[:, 0], SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-13))
+    assert rel(x1, ref[:, 0]) <= 1e-11
+
+
+def test_newmark_initial_acceleration(gpu):
+    from femhip import Equation, HIPSolver, NewmarkParameters
+    m = meshes.jittered_square(30, seed=4)
+    s = oracle.System(m, bc=0)
+    u = s.gather(np.exp(-((m["xyz"][:, 0] - 0.4) ** 2 + (m["xyz"][:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2)))
+    a_ref = s.newmark_init(1.3, u)
+    ctx = FEMContext(m, 0)
+    ctx.equation = Equation.Wave_Newmark
+    ctx.parameters[Equation.Wave_Newmark] = NewmarkParameters(2e-4, 1.3, 0.25, 0.5)
+    sol = HIPSolver(ctx)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    a = sol.init_newmark_acceleration()
+    assert rel(a, a_ref) <= 1e-10
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_A), a_ref) <= 1e-10

@@ -216,6 +216,56 @@ def test_batch_uneven_meshes_multi_rhs(gpu):
             assert rel(got[:, j], u) < RTOL, (i, j)
 
 
+def test_batch_advection_diffusion_16_rhs(gpu):
+    """16 right-hand sides per instance with BiCGSTAB on uneven meshes with
+    per-instance unnormalised velocity and diffusivity."""
+    specs = [(10, 1, (0.6, 0.2, 0.0), 3e-2), (16, 2, (-0.1, 0.9, 0.0), 1e-3), (6, 3, (0.3, -0.3, 0.0), 1e-1)]
+    k, dt, T = 16, 1e-3, 3
+    rng = np.random.default_rng(17)
+    ctxs, systems, states = [], [], []
+    for side, seed, vel, _ in specs:
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+        s = oracle.System(m, bc=0, velocity=vel, normalize=False)
+        systems.append(s)
+        states.append(rng.normal(size=(s.n, k)))
+    b = Batch(ctxs, SolverOptions(nrhs=k))
+    for i, spec in enumerate(specs):
+        b.set_equation(i, AdvectionDiffusionParameters(dt, spec[3]))
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+    print("case batch advdiff 16 rhs", flush=True)
+    assert b.advance(T) == _lib.FEM_OK
+    for i, s in enumerate(systems):
+        got = b.get_state_unknowns(i).reshape(s.n, k)
+        for j in range(k):
+            _, u, _, _ = s.run(oracle.ADVDIFF, [dt, specs[i][3]], T, states[i][:, j].copy(), krylov=oracle.BICGSTAB,
+                               tol=1e-14)
+            assert rel(got[:, j], u) < RTOL, (i, j)
+
+
+@pytest.mark.parametrize("krylov", [_lib.FEM_KRYLOV_CG, _lib.FEM_KRYLOV_BICGSTAB, _lib.FEM_KRYLOV_CG_CLASSIC])
+def test_fenced_exchange_large_row_blocks(gpu, krylov):
+    """Three workgroups over ~2,200 rows x 4 columns: each workgroup streams
+    more rows than its LDS row cache holds, so the launch uses fenced
+    barriers with cached data accesses."""
+    m = meshes.jittered_square(48, seed=6)
+    vel = (0.4, 0.7, 0.0)
+    k, dt, c, T = 4, 1e-3, 2e-2, 4
+    ctx = FEMContext(m, 0, velocity=vel, normalize_velocity=False)
+    s = oracle.System(m, bc=0, velocity=vel, normalize=False)
+    rng = np.random.default_rng(23)
+    x0 = rng.normal(size=(s.n, k))
+    b = Batch([ctx], SolverOptions(nrhs=k, team_size=3, krylov=krylov))
+    b.set_equation(0, AdvectionDiffusionParameters(dt, c))
+    b.set_state_unknowns(0, _lib.FEM_FIELD_U, x0)
+    print("case fenced", krylov, flush=True)
+    assert b.advance(T) == _lib.FEM_OK
+    got = b.get_state_unknowns(0).reshape(s.n, k)
+    for j in range(k):
+        _, u, _, _ = s.run(oracle.ADVDIFF, [dt, c], T, x0[:, j].copy(), krylov=oracle.BICGSTAB, tol=1e-14)
+        assert rel(got[:, j], u) < RTOL, j
+
+
 def test_surface_state_and_instability(gpu):
     m = meshes.structured_square(16)
     ctx = FEMContext(m, 0)
