@@ -466,11 +466,36 @@ def run_batched_c4(args):
     return out
 
 
+def run_spmv(args):
+    """The SpMV HBM-bandwidth line: FP32 single right-hand side on the 1.14 GB
+    CSR (other precisions and column counts under "results")."""
+    sp = spmv_measure(reps=args.steps or 50)
+    head = sp["results"][0]
+    return contract_line("SpMV HBM GB/s", head["achieved_GBs"], "GB/s", args.steps or 50, 1, head["ms"], "f32",
+                         {"workload": sp["matrix"], "global_batch": 1, "parallelism": "single GPU"},
+                         roofline={"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": head["frac"], "traffic": None,
+                                   "kernel": "k_spmv_stream<float,1>", "bytes_per_launch": head["bytes"]},
+                         spmv=sp)
+
+
+def run_adjoint(args):
+    """The adjoint/forward cost line (C5 workload)."""
+    ad = adjoint_measure(nsteps=args.steps or 100)
+    out = contract_line("adjoint/forward cost", ad["adjoint_over_forward"], "ratio", args.steps or 100, 1,
+                        ad["backward_ms"] / (args.steps or 100), "f32+f64 (FP32 BiCGSTAB, FP64 refinement)",
+                        {"workload": ad["workload"], "global_batch": 1, "parallelism": "single GPU"},
+                        adjoint=ad)
+    out["higher_is_better"] = False
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="grayscott", choices=["grayscott", "heat", "wave", "batched"],
-                    help="grayscott: the headline line; heat / wave / batched: the other workloads")
+    ap.add_argument("--workload", default="grayscott",
+                    choices=["grayscott", "heat", "wave", "batched", "spmv", "adjoint"],
+                    help="grayscott: the headline line; the others: one workload each")
     ap.add_argument("--instances", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
@@ -503,7 +528,8 @@ def main():
                                    "(see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload != "grayscott":
-        fn = {"heat": run_heat_c1, "wave": run_wave_c3, "batched": run_batched_c4}[args.workload]
+        fn = {"heat": run_heat_c1, "wave": run_wave_c3, "batched": run_batched_c4, "spmv": run_spmv,
+              "adjoint": run_adjoint}[args.workload]
         print(json.dumps(fn(args)))
         return
     if args.steps is None:

@@ -408,7 +408,10 @@ int build_plan(fem_batch* b) {
       if (rows * k > 2LL * kBlock) streaming = true;
     }
   }
+  const int forced = (b->opts.exchange == FEM_EXCHANGE_COHERENT || b->opts.exchange == FEM_EXCHANGE_FENCED)
+                         ? b->opts.exchange : 0;
   b->launch.coh = !multi ? 0
+                  : forced ? forced
                   : b->mode_override ? b->mode_override
                   : streaming ? kExchangeFenced : kExchangeCoherent;
   b->nteams = nteams;
@@ -829,6 +832,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   if (!rc) {
     for (int c : cands) {
       for (int m : {kExchangeCoherent, kExchangeFenced}) {
+        if (b->opts.exchange != FEM_EXCHANGE_AUTO && m != kExchangeCoherent) continue;  // mode fixed by opts
         if (single) b->opts.team_size = c;
         else b->grid_budget = c;
         b->mode_override = m;

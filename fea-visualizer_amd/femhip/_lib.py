@@ -49,6 +49,9 @@ FEM_KRYLOV_BICGSTAB = 2
 FEM_KRYLOV_CG_CLASSIC = 3
 FEM_REACTION_NODAL = 0
 FEM_REACTION_LUMPED = 1
+FEM_EXCHANGE_AUTO = 0
+FEM_EXCHANGE_COHERENT = 1
+FEM_EXCHANGE_FENCED = 2
 FEM_SPMV_STREAM = 32
 FEM_SPMV_STREAM_NT = 33
 SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)
@@ -113,6 +116,7 @@ class fem_solver_opts(C.Structure):
         ("team_size", C.c_int32),
         ("tol", C.c_double),
         ("inner_tol", C.c_double),
+        ("exchange", C.c_int32),
     ]
 
 

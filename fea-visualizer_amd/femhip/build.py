@@ -64,5 +64,22 @@ def build(out: str = OUT, verbose: bool = False, jobs: int | None = None) -> str
     return out
 
 
+DRIVER_SRC = os.path.join(REPO, "tests", "cpp", "solver_driver.cpp")
+DRIVER = os.path.join(REPO, "tests", "cpp", "solver_driver")
+
+
+def driver_command(out: str = DRIVER, lib_dir: str = HERE):
+    """g++ line of the headless Solver-adapter driver (links libfemhip.so,
+    found at run time relative to the binary)."""
+    rel = os.path.relpath(lib_dir, os.path.dirname(os.path.abspath(out)))
+    return ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", DRIVER_SRC, f"-L{lib_dir}", "-lfemhip",
+            f"-Wl,-rpath,$ORIGIN/{rel}", "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
+
+
+def build_driver(out: str = DRIVER, verbose: bool = False) -> str:
+    _run(driver_command(out), verbose)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -106,6 +106,7 @@ class SolverOptions:
     team_size: int = 0
     tol: float = 1e-10
     inner_tol: float = 1e-4
+    exchange: int = L.FEM_EXCHANGE_AUTO
 
     def to_c(self):
         o = L.fem_solver_opts()

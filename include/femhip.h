@@ -113,7 +113,15 @@ typedef struct fem_solver_opts {
   int32_t team_size;     /* workgroups per instance; 0 = autotuned */
   double tol;            /* relative residual |b - Ax| <= tol |b| in FP64; 0 = 1e-10 */
   double inner_tol;      /* FP32 sweep reduction; 0 = 1e-4 */
+  int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or
+                            autotuner), _COHERENT or _FENCED */
 } fem_solver_opts;
+
+/* How the workgroups of one team exchange vectors between barriers:
+ * COHERENT: device-coherent loads/stores, fence-free barriers (latency-bound
+ * phases); FENCED: cached accesses, L2 write-back/invalidate at barriers
+ * (bandwidth-bound phases). */
+enum { FEM_EXCHANGE_AUTO = 0, FEM_EXCHANGE_COHERENT = 1, FEM_EXCHANGE_FENCED = 2 };
 
 typedef struct fem_step_stats {
   int64_t iterations;    /* Krylov iterations summed over solves and steps */

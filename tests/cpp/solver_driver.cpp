@@ -1,0 +1,101 @@
+// solver_driver.cpp -- headless replay of the reference's frame loop
+// (src/Application.cpp:530-540) through the C++ adapter femhip::HIPSolver:
+// every frame gathers the surface values, advances one step, checks the
+// instability rule (clearing and stopping on it, as the reference pauses),
+// and scatters the values back.
+//
+// usage: solver_driver <in.bin> <out.bin> <equation> <steps> <p0> <p1> [p2 p3 p4]
+//   in.bin : int32 nv, int32 ntri, int32 bc, float xyz[3 nv], uint32 tri[3 ntri],
+//            uint8 on_boundary[nv], float values[nv]
+//   out.bin: int32 frames_run, int32 unstable, float values[nv]
+//   equation: heat | advdiff | wave | grayscott  (parameters in the order of
+//             EquationParameters.hpp; advdiff uses the default velocity)
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "femhip_solver.hpp"
+
+template <typename T>
+static void rd(FILE* f, T* p, size_t n) {
+  if (fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short read");
+}
+
+int main(int argc, char** argv) {
+  if (argc < 7) {
+    fprintf(stderr, "usage: %s in.bin out.bin equation steps p0 p1 [p2 p3 p4]\n", argv[0]);
+    return 2;
+  }
+  try {
+    FILE* f = fopen(argv[1], "rb");
+    if (!f) throw std::runtime_error("cannot open input");
+    int32_t nv = 0, nt = 0, bc = 0;
+    rd(f, &nv, 1);
+    rd(f, &nt, 1);
+    rd(f, &bc, 1);
+    std::vector<float> xyz(3 * (size_t)nv), values(nv);
+    std::vector<uint32_t> tri(3 * (size_t)nt);
+    std::vector<uint8_t> onb(nv);
+    rd(f, xyz.data(), xyz.size());
+    rd(f, tri.data(), tri.size());
+    rd(f, onb.data(), onb.size());
+    rd(f, values.data(), values.size());
+    fclose(f);
+
+    const std::string eq = argv[3];
+    const int steps = atoi(argv[4]);
+    double p[5] = {0, 0, 0, 0, 0};
+    for (int i = 5; i < argc && i < 10; i++) p[i - 5] = atof(argv[i]);
+
+    femhip::Context ctx(xyz.data(), nv, tri.data(), nt, onb.data(), bc);
+    femhip::HIPSolver solver(ctx);
+    if (eq == "heat") {
+      solver.set_equation(femhip::HeatParameters{p[0], p[1]});
+    } else if (eq == "advdiff") {
+      femhip::AdvectionDiffusionParameters a;
+      a.time_step = p[0];
+      a.c = p[1];
+      ctx.assemble(a.velocity);
+      solver.set_equation(a);
+    } else if (eq == "wave") {
+      solver.set_equation(femhip::WaveParameters{p[0], p[1]});
+    } else if (eq == "grayscott") {
+      femhip::ReactionDiffusionParameters r;
+      r.time_step = p[0];
+      r.Du = p[1];
+      r.Dv = p[2];
+      r.feed_rate = p[3];
+      r.kill_rate = p[4];
+      solver.set_equation(r);
+    } else {
+      throw std::runtime_error("unknown equation " + eq);
+    }
+    solver.attach_surface_values(values.data());
+    int frames = 0, unstable = 0;
+    for (; frames < steps; frames++) {
+      solver.advance_time();
+      if (solver.has_numerical_instability()) {
+        // Application::clear_solver (Application.cpp:579-583): solver and surface values
+        solver.clear_values();
+        std::fill(values.begin(), values.end(), 0.0f);
+        unstable = 1;
+        frames++;
+        break;
+      }
+    }
+    FILE* o = fopen(argv[2], "wb");
+    if (!o) throw std::runtime_error("cannot open output");
+    fwrite(&frames, sizeof(int32_t), 1, o);
+    fwrite(&unstable, sizeof(int32_t), 1, o);
+    fwrite(values.data(), sizeof(float), values.size(), o);
+    fclose(o);
+    printf("frames %d unstable %d unknowns %u\n", frames, unstable, ctx.num_unknowns());
+  } catch (const std::exception& e) {
+    fprintf(stderr, "solver_driver: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

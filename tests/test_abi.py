@@ -34,7 +34,7 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_lib.fem_mesh) == 32
     assert C.sizeof(_lib.fem_assembly_params) == 48
     assert C.sizeof(_lib.fem_rd_params) == 48
-    assert C.sizeof(_lib.fem_solver_opts) == 56
+    assert C.sizeof(_lib.fem_solver_opts) == 64
     assert C.sizeof(_lib.fem_step_stats) == 48
     assert C.sizeof(_lib.fem_ctx_info) == 24
 
