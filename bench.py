@@ -32,6 +32,13 @@ for _p in (ROOT, os.path.join(ROOT, "fea-visualizer_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM-side bytes measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
+# against a plain copy kernel) for the dominant kernel of a workload; the
+# summaries with their commands are in profiles/.
+PROFILED_TRAFFIC = {
+    "grayscott": {"bytes_per_step": 8.529e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
+    "spmv": {"bytes_per_launch": 1.2095e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
+}
 METRIC = "DOF·steps/s at rel-L2≤1e-8 vs FP64"
 
 
@@ -474,7 +481,9 @@ def run_spmv(args):
     return contract_line("SpMV HBM GB/s", head["achieved_GBs"], "GB/s", args.steps or 50, 1, head["ms"], "f32",
                          {"workload": sp["matrix"], "global_batch": 1, "parallelism": "single GPU"},
                          roofline={"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                                   "unit": "GB/s", "frac": head["frac"], "traffic": None,
+                                   "unit": "GB/s", "frac": head["frac"],
+                                   "traffic": PROFILED_TRAFFIC["spmv"]["bytes_per_launch"],
+                                   "traffic_source": PROFILED_TRAFFIC["spmv"]["source"],
                                    "kernel": "k_spmv_stream<float,1>", "bytes_per_launch": head["bytes"]},
                          spmv=sp)
 
@@ -601,8 +610,10 @@ def main():
     # roofline of the persistent stepping kernel (one launch = the timed region)
     bytes_alg = step_bytes(n, nnz, 2, 2, st.iterations, st.sweeps, args.steps)
     ach = bytes_alg / (kernel_ms * 1e-3) / 1e9
+    tr = PROFILED_TRAFFIC["grayscott"]
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "frac": ach / HBM_PEAK_GBS, "traffic": tr["bytes_per_step"] * args.steps,
+                       "traffic_source": tr["source"] + " (per-step bytes x steps)",
                        "kernel": "k_step<2,float>", "bytes_per_launch": bytes_alg,
                        "kernel_ms": kernel_ms}
 
