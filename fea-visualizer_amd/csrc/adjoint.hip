@@ -96,8 +96,8 @@ int adjoint_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_adjoint** out)
   fem_solver_opts o;
   fem_opts_default(&o);
   if (opts) o = *opts;
-  if (o.engine != FEM_ENGINE_PERSISTENT) {
-    set_error("the adjoint runs on the persistent engine");
+  if (o.engine == FEM_ENGINE_PLAIN) {
+    set_error("the adjoint runs on the persistent or streaming engine");
     return FEM_ERR_INVALID;
   }
   if (o.nrhs > 1) {

@@ -409,8 +409,8 @@ int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opt
       set_error("fem_ctx_solve: null argument");
       return FEM_ERR_INVALID;
     }
-    if (opts && opts->engine != FEM_ENGINE_PERSISTENT) {
-      set_error("fem_ctx_solve runs on the persistent engine");
+    if (opts && opts->engine == FEM_ENGINE_PLAIN) {
+      set_error("fem_ctx_solve runs on the persistent or streaming engine");
       return FEM_ERR_INVALID;
     }
     return linear_solve(ctx, coef, opts, b, x, stats);

@@ -429,6 +429,8 @@ void batch_free(fem_batch* b) {
   for (auto& h : b->insts) free_inst(h);
   free_plan(b);
   if (b->d_plain) (void)hipFree(b->d_plain);
+  if (b->splan) stream_plan_free(b->splan);
+  if (b->d_sinsts) (void)hipFree(b->d_sinsts);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -453,6 +455,12 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
   if (b->opts.precision != FEM_PREC_FP64 && b->opts.precision != FEM_PREC_FP32_IR) {
     delete b;
     set_error("unknown precision");
+    return FEM_ERR_INVALID;
+  }
+  if (b->opts.engine != FEM_ENGINE_PERSISTENT && b->opts.engine != FEM_ENGINE_PLAIN &&
+      b->opts.engine != FEM_ENGINE_STREAM) {
+    delete b;
+    set_error("unknown engine");
     return FEM_ERR_INVALID;
   }
   b->nrhs = nr;
@@ -760,6 +768,7 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     return FEM_ERR_INVALID;
   }
   if (b->opts.engine == FEM_ENGINE_PLAIN) return plain_advance(b, nsteps, stats);
+  if (b->opts.engine == FEM_ENGINE_STREAM) return stream_advance(b, nsteps, stats);
   int rc = batch_prepare(b, true);
   if (rc) return rc;
   const int ninst = (int)b->insts.size();
@@ -783,6 +792,49 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     return FEM_ERR_TIMEOUT;
   }
   return rc;
+}
+
+// Streaming engine: every phase a launch over the row blocks of all
+// instances (stream.hip).  Instances must share the column count, the inner
+// precision and the Krylov method, as in the persistent plan.
+int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
+  int rc = batch_prepare(b, false);
+  if (rc) return rc;
+  const int ninst = (int)b->insts.size();
+  const int k = b->insts[0].k, kr = b->insts[0].dev.krylov;
+  for (auto& h : b->insts)
+    if (h.k != k || h.dev.krylov != kr) {
+      set_error("batch instances must share the number of columns and the Krylov method");
+      return FEM_ERR_INVALID;
+    }
+  const int fp32 = inner_bytes(b) == 4;
+  if (b->sinsts_n != ninst) {
+    if (b->d_sinsts) (void)hipFree(b->d_sinsts);
+    b->d_sinsts = dalloc<Inst>(ninst, &rc);
+    if (rc) return rc;
+    b->sinsts_n = ninst;
+  }
+  std::vector<Inst> hi(ninst);
+  for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
+  FEM_HIP(hipMemcpyAsync(b->d_sinsts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
+  const int rows = b->stream_rows > 0 ? b->stream_rows : (k >= 16 ? 64 : 256);
+  if (!b->splan || b->splan_key != (long long)rows * 1000003LL + k * 131 + kr * 7 + fp32) {
+    if (b->splan) stream_plan_free(b->splan);
+    b->splan = nullptr;
+    rc = stream_plan_build(b->insts, b->d_sinsts, k, fp32, kr, rows, b->stream, &b->splan);
+    if (rc) return rc;
+    b->splan_key = (long long)rows * 1000003LL + k * 131 + kr * 7 + fp32;
+  }
+  const int max_sweeps = b->opts.max_sweeps > 0 ? b->opts.max_sweeps : 30;
+  const int max_inner = b->opts.max_inner > 0 ? b->opts.max_inner : 200;
+  FEM_HIP(hipEventRecord(b->ev0, b->stream));
+  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream);
+  if (rc) return rc;
+  FEM_HIP(hipEventRecord(b->ev1, b->stream));
+  FEM_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  return collect_stats(b, stats, ms);
 }
 
 // Autotuner: run `probe` steps under a few launch plans from the same state

@@ -24,6 +24,14 @@ struct StepLaunch {
 constexpr int kExchangeCoherent = 1, kExchangeFenced = 2;
 
 int launch_step(const StepLaunch& L, int nsteps, hipStream_t st);
+
+// Streaming engine (stream.hip): row-block table and per-instance scalars.
+struct StreamPlan;
+struct InstHost;
+void stream_plan_free(StreamPlan* p);
+int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
+                      hipStream_t st, StreamPlan** out);
+int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps, hipStream_t st);
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
 
 // Per-instance host bookkeeping.
@@ -82,6 +90,12 @@ struct fem_batch {
   // plain-engine scratch
   double* d_plain = nullptr;
   long long plain_n = 0;
+  // streaming engine
+  femhip::StreamPlan* splan = nullptr;
+  femhip::Inst* d_sinsts = nullptr;
+  int sinsts_n = 0;
+  int stream_rows = 0;  // rows per block (0 = by column count)
+  long long splan_key = -1;
 };
 
 struct fem_solver {
@@ -99,6 +113,7 @@ int batch_clear(fem_batch* b, int inst);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
+int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_set_linear(fem_batch* b, int inst, const double coef[3]);
 int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                  fem_step_stats* stats);

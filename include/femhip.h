@@ -59,7 +59,11 @@ enum { FEM_FIELD_U = 0, FEM_FIELD_V = 1, FEM_FIELD_A = 2 };
 enum { FEM_MATRIX_K = 0, FEM_MATRIX_M = 1, FEM_MATRIX_A = 2 };
 
 /* Solver engines and arithmetic. */
-enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1 };
+/* PERSISTENT: one cooperative launch per advance (latency-bound instances);
+ * PLAIN: FP64 baseline, one right-hand side at a time with host-read scalars;
+ * STREAM: each phase a full-occupancy launch over the row blocks of all
+ * instances (bandwidth-bound instances: large meshes, many right-hand sides). */
+enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1, FEM_ENGINE_STREAM = 2 };
 enum { FEM_PREC_FP64 = 0, FEM_PREC_FP32_IR = 1 };
 /* FEM_KRYLOV_CG is single-reduction (Chronopoulos-Gear) PCG;
  * FEM_KRYLOV_CG_CLASSIC the two-reduction form of Eigen's conjugate_gradient. */

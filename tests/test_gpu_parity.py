@@ -338,3 +338,99 @@ def test_team_sizes_and_reduction_modes(gpu, team):
     assert sol.advance_time(6) == _lib.FEM_OK
     assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
     assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+
+
+STREAM_CASES = ["heat_cg", "heat_classic", "advdiff_batch16", "grayscott", "newmark", "wave_fp64"]
+
+
+@pytest.mark.parametrize("case", STREAM_CASES)
+def test_stream_engine_matches_oracle(gpu, case):
+    """The streaming engine (one launch per phase over all row blocks) against
+    the oracle, for every Krylov method, several equations and a batch."""
+    print("case stream", case, flush=True)
+    rng = np.random.default_rng(31)
+    if case == "advdiff_batch16":
+        specs = [(18, 1, (0.6, 0.2, 0.0), 3e-2), (9, 2, (-0.1, 0.9, 0.0), 1e-3), (25, 3, (0.3, -0.3, 0.0), 1e-1)]
+        k, dt, T = 16, 1e-3, 3
+        ctxs, systems, states = [], [], []
+        for side, seed, vel, _ in specs:
+            m = meshes.jittered_square(side, seed=seed)
+            ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+            systems.append(oracle.System(m, bc=0, velocity=vel, normalize=False))
+            states.append(rng.normal(size=(systems[-1].n, k)))
+        b = Batch(ctxs, SolverOptions(nrhs=k, engine=_lib.FEM_ENGINE_STREAM))
+        for i, spec in enumerate(specs):
+            b.set_equation(i, AdvectionDiffusionParameters(dt, spec[3]))
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+        assert b.advance(T) == _lib.FEM_OK
+        for i, s in enumerate(systems):
+            got = b.get_state_unknowns(i).reshape(s.n, k)
+            for j in range(k):
+                _, u, _, _ = s.run(oracle.ADVDIFF, [dt, specs[i][3]], T, states[i][:, j].copy(),
+                                   krylov=oracle.BICGSTAB, tol=1e-14)
+                assert rel(got[:, j], u) < RTOL, (i, j)
+        return
+    if case == "grayscott":
+        m = meshes.icosphere(3)
+        s = oracle.System(m, bc=1)
+        u = np.ones(s.n)
+        v = np.where(rng.uniform(size=s.n) < 0.05, 0.25, 0.0)
+        prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+        ctx = FEMContext(m, 1)
+        sol = make_solver(ctx, Equation.Reaction_Diffusion, prm, engine=_lib.FEM_ENGINE_STREAM)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        sol.set_state_unknowns(_lib.FEM_FIELD_V, v)
+        for _ in range(5):
+            _, u, v, _ = s.step(oracle.GRAYSCOTT, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], u, v, tol=1e-14)
+        assert sol.advance_time(5) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+        return
+    m = meshes.jittered_square(40, seed=8)
+    s = oracle.System(m, bc=0)
+    u = s.gather(gauss(m, (0.45, 0.5), 0.02))
+    ctx = FEMContext(m, 0)
+    if case == "newmark":
+        a = s.newmark_init(1.0, u)
+        sol = make_solver(ctx, Equation.Wave_Newmark, NewmarkParameters(2e-3, 1.0, 0.25, 0.5),
+                          engine=_lib.FEM_ENGINE_STREAM)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        sol.set_state_unknowns(_lib.FEM_FIELD_A, a)
+        v = np.zeros(s.n)
+        for _ in range(6):
+            _, u, v, a = s.step(oracle.NEWMARK, [2e-3, 1.0, 0.25, 0.5], u, v, a, tol=1e-14)
+        assert sol.advance_time(6) == _lib.FEM_OK
+    elif case == "wave_fp64":
+        sol = make_solver(ctx, Equation.Wave, WaveParameters(5e-3, 0.8), engine=_lib.FEM_ENGINE_STREAM,
+                          precision=_lib.FEM_PREC_FP64)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        v = np.zeros(s.n)
+        for _ in range(6):
+            _, u, v, _ = s.step(oracle.WAVE, [5e-3, 0.8], u, v, tol=1e-14)
+        assert sol.advance_time(6) == _lib.FEM_OK
+    else:
+        kr = _lib.FEM_KRYLOV_CG if case == "heat_cg" else _lib.FEM_KRYLOV_CG_CLASSIC
+        sol = make_solver(ctx, Equation.Heat, HeatParameters(1e-3, 1.0), engine=_lib.FEM_ENGINE_STREAM, krylov=kr)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        _, u, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], 6, u.copy(), tol=1e-14)
+        assert sol.advance_time(6) == _lib.FEM_OK
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+
+
+def test_stream_engine_adjoint_matches_persistent(gpu):
+    from femhip import Adjoint
+    m = meshes.jittered_square(24, seed=12)
+    ctx = FEMContext(m, 0, velocity=(0.4, 0.2, 0.0), normalize_velocity=False)
+    rng = np.random.default_rng(4)
+    u0 = rng.normal(size=ctx.num_unknowns())
+    g = rng.normal(size=ctx.num_unknowns())
+    out = []
+    for eng in (_lib.FEM_ENGINE_PERSISTENT, _lib.FEM_ENGINE_STREAM):
+        adj = Adjoint(ctx, AdvectionDiffusionParameters(2e-3, 1e-2),
+                      SolverOptions(engine=eng, precision=_lib.FEM_PREC_FP64, tol=1e-13))
+        uT = adj.forward(u0, 7, 3)
+        gu, gk = adj.backward(g)
+        out.append((uT, gu, gk))
+        adj.close()
+    for a_, b_ in zip(out[0], out[1]):
+        assert rel(b_, a_) < 1e-10
