@@ -301,6 +301,7 @@ int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr) {
     int rc = batch_field_ptr(s->batch, 0, field, &d, &col, &nc);
     if (rc) return rc;
     *dptr = (void*)d;
+    s->batch->insts[0].hist = false;  // the caller may write the state: restart the predictor
     return FEM_OK;
   })
 }

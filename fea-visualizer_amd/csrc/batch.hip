@@ -244,6 +244,10 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.last_res = h.d_last_res;
   I.traj = h.traj;
   I.traj_every = h.traj ? std::max(1, h.traj_every) : 1;
+  I.extrap = (o.warm_start && o.predictor == FEM_PREDICTOR_LINEAR &&
+              (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION ||
+               h.eq == FEM_WAVE) && h.eq_override < 0) ? 1 : 0;
+  I.hist = (I.extrap && h.hist) ? 1 : 0;
 }
 
 void free_plan(fem_batch* b) {
@@ -549,8 +553,10 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
       if (rc) return rc;
       b->plan_ready = false;
     }
-    if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w)
+    if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;
+      h.hist = false;
+    }
     if (h.eq != eq) b->plan_ready = false;
     h.eq = eq;
     h.dt = dt;
@@ -666,6 +672,7 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
   hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
   FEM_HIP(hipStreamSynchronize(b->stream));
   (void)hipFree(tmp);
+  h.hist = false;  // the predictor's history no longer matches the state
   return 0;
 }
 
@@ -697,6 +704,7 @@ int batch_clear(fem_batch* b, int inst) {
     FEM_HIP(hipMemsetAsync(h.d_a, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_x, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_status, 0, sizeof(int), b->stream));
+    h.hist = false;
   }
   FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
@@ -788,9 +796,12 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
   rc = collect_stats(b, stats, ms);
   if (abort_flag) {
+    for (auto& h : b->insts) h.hist = false;
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
+  if (nsteps > 0)
+    for (auto& h : b->insts) h.hist = true;
   return rc;
 }
 
@@ -834,7 +845,10 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipEventSynchronize(b->ev1));
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  return collect_stats(b, stats, ms);
+  rc = collect_stats(b, stats, ms);
+  if (rc >= 0 && nsteps > 0)
+    for (auto& h : b->insts) h.hist = true;
+  return rc;
 }
 
 // Autotuner: run `probe` steps under a few launch plans from the same state
@@ -870,11 +884,14 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
       snap.push_back(d);
     }
   }
+  std::vector<char> hist0;
+  for (auto& h : b->insts) hist0.push_back(h.hist ? 1 : 0);
   auto restore = [&]() -> int {
-    size_t q = 0;
+    size_t q = 0, i = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
       for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x}) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
+      h.hist = hist0[i++] != 0;
     }
     return 0;
   };
@@ -905,6 +922,21 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
       if (rc) break;
     }
   }
+  // the streaming engine as one more candidate
+  bool stream_best = false;
+  if (!rc) {
+    b->opts.engine = FEM_ENGINE_STREAM;
+    if (!(rc = restore())) {
+      fem_step_stats st{};
+      const int r = batch_advance(b, probe, &st);
+      if (r < 0) rc = r;
+      else if (st.ms < best) {
+        best = st.ms;
+        stream_best = true;
+      }
+    }
+    b->opts.engine = FEM_ENGINE_PERSISTENT;
+  }
   const int rr = restore();
   if (!rc) rc = rr;
   FEM_HIP(hipStreamSynchronize(b->stream));
@@ -912,8 +944,9 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   if (single) b->opts.team_size = rc ? saved_team : best_c;
   else b->grid_budget = rc ? saved_budget : best_c;
   b->mode_override = rc ? saved_mode : best_m;
+  if (!rc && stream_best) b->opts.engine = FEM_ENGINE_STREAM;
   b->plan_ready = false;
-  if (chosen) *chosen = best_c;
+  if (chosen) *chosen = (!rc && stream_best) ? 0 : best_c;
   if (best_ms) *best_ms = best;
   return rc;
 }

@@ -89,6 +89,8 @@ struct Inst {
   double* last_res;      // [k] final relative residual of the last solve
   double* traj;          // optional: primary variable after every traj_every-th
   int traj_every;        //   step of a launch, at traj[(step + 1) / traj_every - 1][n][k]
+  int extrap;            // initial guess 2 x_n - x_{n-1} (x_{n-1} kept in `a`)
+  int hist;              // `a` holds x_{n-1} at launch start
   int pad2_;
 };
 

@@ -356,7 +356,7 @@ __device__ inline A row_gather(int lo, int hi, A zero, F f) {
 }
 
 template <int K, typename TI, int C>
-__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, double (&acc)[3]) {
+__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step, double (&acc)[3]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -364,6 +364,8 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, double (
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
+  // predictor: the guess 2 x_n - x_{n-1} once the previous solution is known
+  const bool ex = warm && I.extrap && (step > 0 || I.hist);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -375,37 +377,38 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, double (
     double b = 0.0, sx = 0.0, x0 = 0.0;
     switch (I.eq) {
       case EQ_HEAT:
-      case EQ_ADVDIFF: {  // b = (M/dt) u                       CPUSolver.cpp:78,96
+      case EQ_ADVDIFF:
+      case EQ_GS: {  // b = (M/dt) u                                  CPUSolver.cpp:78,96
+        // Gray-Scott: b_u = (M/dt)u - uvv + F(1-u); b_v = (M/dt)v + uvv - (F+k)v   CPUSolver.cpp:134,137
         const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
-          const double uc = ld<C>(I.u + (size_t)ci[s] * K + jr);
-          return D2{M64[s] * uc, warm ? S64[(size_t)s * kv + vj] * uc : 0.0};
+          const size_t oc = (size_t)ci[s] * K + jr;
+          const double uc = ld<C>(I.u + oc);
+          const double xc = ex ? 2.0 * uc - ld<C>(I.a + oc) : uc;
+          return D2{M64[s] * uc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
         });
         b = g.a * inv_dt;
+        if (I.eq == EQ_GS) {
+          const double uu = ld<C>(I.u + (size_t)row * K), vv = ld<C>(I.u + (size_t)row * K + 1);
+          const double uvv = uu * (vv * vv);
+          const double f = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
+          const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
+          b += w * f;
+        }
         sx = g.b;
-        x0 = warm ? ld<C>(I.u + o) : 0.0;
+        const double uo = ld<C>(I.u + o);
+        x0 = warm ? (ex ? 2.0 * uo - ld<C>(I.a + o) : uo) : 0.0;
       } break;
       case EQ_WAVE: {  // b = (M/dt) v - c^2 K u                    CPUSolver.cpp:114
         const D3 g = row_gather(lo, hi, D3{0.0, 0.0, 0.0}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + jr;
           const double vc = ld<C>(I.v + oc);
-          return D3{M64[s] * vc, K64[s] * ld<C>(I.u + oc), warm ? S64[(size_t)s * kv + vj] * vc : 0.0};
+          const double xc = ex ? 2.0 * vc - ld<C>(I.a + oc) : vc;
+          return D3{M64[s] * vc, K64[s] * ld<C>(I.u + oc), warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
         });
         b = g.a * inv_dt - I.p1 * I.p1 * g.b;
         sx = g.c;
-        x0 = warm ? ld<C>(I.v + o) : 0.0;
-      } break;
-      case EQ_GS: {  // b_u = (M/dt)u - uvv + F(1-u); b_v = (M/dt)v + uvv - (F+k)v   CPUSolver.cpp:134,137
-        const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
-          const double yc = ld<C>(I.u + (size_t)ci[s] * K + jr);
-          return D2{M64[s] * yc, warm ? S64[(size_t)s * kv + vj] * yc : 0.0};
-        });
-        const double uu = ld<C>(I.u + (size_t)row * K), vv = ld<C>(I.u + (size_t)row * K + 1);
-        const double uvv = uu * (vv * vv);
-        const double f = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
-        const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
-        b = g.a * inv_dt + w * f;
-        sx = g.b;
-        x0 = warm ? ld<C>(I.u + o) : 0.0;
+        const double vo = ld<C>(I.v + o);
+        x0 = warm ? (ex ? 2.0 * vo - ld<C>(I.a + o) : vo) : 0.0;
       } break;
       case EQ_NEWMARK: {  // b = -c^2 K (u + dt v + (1/2 - beta) dt^2 a)
         const double h = (0.5 - I.p2) * I.dt * I.dt;
@@ -919,9 +922,11 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS:
+        if (I.extrap) st<C>(I.a + o, ld<C>(I.u + o));  // x_{n-1} for the predictor
         st<C>(I.u + o, xf);
         break;
       case EQ_WAVE: {
+        if (I.extrap) st<C>(I.a + o, ld<C>(I.v + o));
         st<C>(I.v + o, xf);
         const double un = ld<C>(I.u + o) + xf * I.dt;  // CPUSolver.cpp:118
         st<C>(I.u + o, un);
@@ -1168,7 +1173,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     // ---- R: right-hand side and initial residual
     {
       double acc[3] = {0.0, 0.0, 0.0};
-      phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, acc);
+      phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
     }
     if (threadIdx.x < K) {

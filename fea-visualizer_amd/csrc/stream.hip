@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   const int par = a.par;
   TI* r0 = (TI*)I.r;
   if constexpr (PH == PH_RHS) {
-    phase_rhs<K, TI, kPlain>(I, row_lo, row_hi, r0, acc);
+    phase_rhs<K, TI, kPlain>(I, row_lo, row_hi, r0, a.step, acc);
   } else if constexpr (PH == PH_RES) {
     phase_res<K, TI, kPlain>(I, row_lo, row_hi, r0, sh, acc);
   } else if constexpr (PH == PH_PCG_INIT) {

@@ -53,6 +53,8 @@ FEM_REACTION_LUMPED = 1
 FEM_EXCHANGE_AUTO = 0
 FEM_EXCHANGE_COHERENT = 1
 FEM_EXCHANGE_FENCED = 2
+FEM_PREDICTOR_LINEAR = 0
+FEM_PREDICTOR_PREVIOUS = 1
 FEM_SPMV_STREAM = 32
 FEM_SPMV_STREAM_NT = 33
 SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)
@@ -118,6 +120,7 @@ class fem_solver_opts(C.Structure):
         ("tol", C.c_double),
         ("inner_tol", C.c_double),
         ("exchange", C.c_int32),
+        ("predictor", C.c_int32),
     ]
 
 

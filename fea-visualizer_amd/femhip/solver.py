@@ -107,6 +107,7 @@ class SolverOptions:
     tol: float = 1e-10
     inner_tol: float = 1e-4
     exchange: int = L.FEM_EXCHANGE_AUTO
+    predictor: int = L.FEM_PREDICTOR_LINEAR
 
     def to_c(self):
         o = L.fem_solver_opts()
@@ -318,6 +319,8 @@ class HIPSolver:
         c = C.c_int32()
         ms = C.c_double()
         L.check(self._lib.fem_solver_autotune(self.handle, int(probe_steps), C.byref(c), C.byref(ms)))
+        if c.value == 0:
+            self.options.engine = L.FEM_ENGINE_STREAM
         self.options.team_size = c.value
         return c.value, ms.value
 
@@ -375,6 +378,8 @@ class Batch:
         c = C.c_int32()
         ms = C.c_double()
         L.check(self._lib.fem_batch_autotune(self.handle, int(probe_steps), C.byref(c), C.byref(ms)))
+        if c.value == 0:
+            self.options.engine = L.FEM_ENGINE_STREAM
         return c.value, ms.value
 
     def close(self):

@@ -119,7 +119,11 @@ typedef struct fem_solver_opts {
   double inner_tol;      /* FP32 sweep reduction; 0 = 1e-4 */
   int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or
                             autotuner), _COHERENT or _FENCED */
+  int32_t predictor;     /* initial guess with warm_start: FEM_PREDICTOR_LINEAR (default,
+                            2 x_n - x_{n-1}) or FEM_PREDICTOR_PREVIOUS (x_n) */
 } fem_solver_opts;
+
+enum { FEM_PREDICTOR_LINEAR = 0, FEM_PREDICTOR_PREVIOUS = 1 };
 
 /* How the workgroups of one team exchange vectors between barriers:
  * COHERENT: device-coherent loads/stores, fence-free barriers (latency-bound
@@ -200,8 +204,10 @@ int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr);
 /* Synchronise the solver stream. */
 int fem_solver_sync(fem_solver* s);
 /* Autotuner (no reference counterpart): runs probe_steps under a few launch
- * plans from the current state, restores the state after each, and keeps the
- * fastest.  *chosen receives the team size (workgroups per instance). */
+ * plans from the current state -- persistent team sizes in both exchange
+ * modes, and the streaming engine -- restores the state after each, and keeps
+ * the fastest.  *chosen receives the team size (workgroups per instance), or
+ * 0 when the streaming engine won. */
 int fem_solver_autotune(fem_solver* s, int32_t probe_steps, int32_t* chosen, double* best_ms);
 
 /* ---- batch: many contexts (meshes of uneven size) advanced in one
@@ -214,7 +220,8 @@ int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, 
 int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats);
 int fem_batch_sync(fem_batch* b);
 /* Autotuner over the number of workgroups the batch plan spreads over
- * (*chosen); state restored as for fem_solver_autotune. */
+ * (*chosen; 0 when the streaming engine won); state restored as for
+ * fem_solver_autotune. */
 int fem_batch_autotune(fem_batch* b, int32_t probe_steps, int32_t* chosen, double* best_ms);
 
 /* ---- stand-alone system operator: S = coef[0] M + coef[1] K + coef[2] A of
