@@ -418,6 +418,44 @@ int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opt
   })
 }
 
+int fem_ctx_smooth_normals(fem_ctx* ctx, const float* normals_nv3, const float* values_nv, float extrusion,
+                           float* out_nv3) {
+  FEM_GUARD({
+    if (!ctx || (ctx->nv > 0 && (!normals_nv3 || !values_nv || !out_nv3))) {
+      set_error("fem_ctx_smooth_normals: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return ctx_smooth_normals(ctx, normals_nv3, values_nv, extrusion, out_nv3);
+  })
+}
+
+int fem_solver_smooth_normals(fem_solver* s, const float* normals_nv3, float extrusion, float* out_nv3) {
+  FEM_GUARD({
+    if (!s || !normals_nv3 || !out_nv3) {
+      set_error("fem_solver_smooth_normals: null argument");
+      return FEM_ERR_INVALID;
+    }
+    fem_batch* b = s->batch;
+    fem_ctx* c = b->insts[0].ctx;
+    const size_t nv = (size_t)c->nv;
+    if (nv == 0) return FEM_OK;
+    int rc = 0;
+    float* d = dalloc<float>(7 * nv, &rc);
+    if (!d) return rc;
+    float *dn = d, *dv = d + 3 * nv, *dout = d + 4 * nv;
+    hipError_t e = hipMemcpyAsync(dn, normals_nv3, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+      rc = batch_surface_values(b, 0, dv);
+      if (!rc) rc = ctx_smooth_normals_device(c, dn, dv, extrusion, dout, b->stream);
+      if (!rc) e = hipMemcpyAsync(out_nv3, dout, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost, b->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "fem_solver_smooth_normals");
+    return rc;
+  })
+}
+
 // ----------------------------------------------------------------- operator
 
 int fem_operator_create(fem_ctx* ctx, const double coef[3], int32_t precision, int32_t transpose,

@@ -627,6 +627,34 @@ int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, 
   return out;
 }
 
+// The surface field of the current equation (u; Gray-Scott: v) as float
+// values per vertex, boundary vertices 0 (CPUSolver.cpp:28-37).
+__global__ void k_state_to_surface(const int* idx, int nv, const double* state, int k, int col, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int r = idx[i];
+  out[i] = r >= 0 ? (float)state[(size_t)r * k + col] : 0.0f;
+}
+
+int batch_surface_values(fem_batch* b, int inst, float* d_out) {
+  InstHost& h = b->insts[inst];
+  if (!h.d_u || h.eq < 0) {
+    set_error("surface values: set_equation first");
+    return FEM_ERR_STATE;
+  }
+  double* d;
+  int col, nc;
+  const int field = h.eq == FEM_REACTION_DIFFUSION ? FEM_FIELD_V : FEM_FIELD_U;
+  int rc = batch_field_ptr(b, inst, field, &d, &col, &nc);
+  if (rc) return rc;
+  fem_ctx* c = h.ctx;
+  if (c->nv == 0) return 0;
+  hipLaunchKernelGGL(k_state_to_surface, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, b->stream, c->d_idx, c->nv, d,
+                     h.k, col, d_out);
+  FEM_HIP(hipGetLastError());
+  return 0;
+}
+
 // Map a state field to (device array, first column, column count).
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* ncols) {
   if (inst < 0 || inst >= (int)b->insts.size()) {

@@ -364,3 +364,67 @@ int ctx_create(const fem_mesh* mesh, int bc, fem_ctx** out) {
 }
 
 }  // namespace femhip
+
+namespace femhip {
+namespace {
+
+// smooth_normals.glsl:22-57 without atomics: each vertex sums, over its
+// incident triangles in ascending order, the unnormalised normal of the
+// triangle whose vertices are moved along their normals by
+// min(1, extrusion * max(0, value)).  FP32 as the shader.
+__global__ void k_smooth_normals(int nv, const float* xyz, const uint32_t* tri, const int* inc_ptr, const int* inc,
+                                 const float* nrm, const float* val, float extrusion, float* out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int e = inc_ptr[v]; e < inc_ptr[v + 1]; e++) {
+    const uint32_t* t = tri + 3 * (size_t)inc[e];
+    float p[3][3];
+    for (int a = 0; a < 3; a++) {
+      const size_t w = t[a];
+      const float h = fminf(1.0f, extrusion * fmaxf(0.0f, val[w]));
+      for (int q = 0; q < 3; q++) p[a][q] = xyz[3 * w + q] + h * nrm[3 * w + q];
+    }
+    const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+    const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+    s0 += e1[1] * e2[2] - e1[2] * e2[1];
+    s1 += e1[2] * e2[0] - e1[0] * e2[2];
+    s2 += e1[0] * e2[1] - e1[1] * e2[0];
+  }
+  out[3 * (size_t)v] = s0;
+  out[3 * (size_t)v + 1] = s1;
+  out[3 * (size_t)v + 2] = s2;
+}
+
+}  // namespace
+
+// normals/out: device [nv][3]; values: device [nv].
+int ctx_smooth_normals_device(fem_ctx* c, const float* d_nrm, const float* d_val, float extrusion, float* d_out,
+                              hipStream_t st) {
+  if (c->nv == 0) return 0;
+  hipLaunchKernelGGL(k_smooth_normals, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, st, c->nv, c->d_xyz, c->d_tri,
+                     c->d_inc_ptr, c->d_inc, d_nrm, d_val, extrusion, d_out);
+  FEM_HIP(hipGetLastError());
+  return 0;
+}
+
+int ctx_smooth_normals(fem_ctx* c, const float* nrm, const float* val, float extrusion, float* out) {
+  const size_t nv = (size_t)c->nv;
+  if (nv == 0) return 0;
+  int rc = 0;
+  float* d = dalloc<float>(7 * nv, &rc);  // normals, values, output
+  if (!d) return rc;
+  float *dn = d, *dv = d + 3 * nv, *dout = d + 4 * nv;
+  hipError_t e = hipMemcpyAsync(dn, nrm, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, val, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    rc = ctx_smooth_normals_device(c, dn, dv, extrusion, dout, c->stream);
+    if (!rc) e = hipMemcpyAsync(out, dout, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "smooth_normals");
+  return rc;
+}
+
+}  // namespace femhip

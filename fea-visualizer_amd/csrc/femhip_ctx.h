@@ -49,6 +49,11 @@ int ctx_create(const fem_mesh* mesh, int bc, fem_ctx** out);
 int ctx_assemble(fem_ctx* c, const fem_assembly_params* prm);
 void ctx_free(fem_ctx* c);
 
+// smoothed vertex normals of the extruded surface (ctx.hip)
+int ctx_smooth_normals(fem_ctx* c, const float* nrm, const float* val, float extrusion, float* out);
+int ctx_smooth_normals_device(fem_ctx* c, const float* d_nrm, const float* d_val, float extrusion, float* d_out,
+                              hipStream_t st);
+
 // stand-alone CSR operator (operator.hip)
 int operator_create(fem_ctx* c, const double coef[3], int precision, int transpose, fem_operator** out);
 void operator_free(fem_operator* op);

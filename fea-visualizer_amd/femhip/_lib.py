@@ -165,6 +165,7 @@ SIGNATURES = [
     ("fem_ctx_get_pattern", C.c_int, [P, PI, PI]),
     ("fem_ctx_get_matrix", C.c_int, [P, I32, PD]),
     ("fem_ctx_solve", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
+    ("fem_ctx_smooth_normals", C.c_int, [P, PF, PF, C.c_float, PF]),
     ("fem_solver_create", C.c_int, [P, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_solver_destroy", C.c_int, [P]),
     ("fem_solver_set_equation", C.c_int, [P, I32, P, C.c_size_t]),
@@ -177,6 +178,7 @@ SIGNATURES = [
     ("fem_solver_unstable", C.c_int, [P, PI]),
     ("fem_solver_state_device_ptr", C.c_int, [P, I32, C.POINTER(P)]),
     ("fem_solver_sync", C.c_int, [P]),
+    ("fem_solver_smooth_normals", C.c_int, [P, PF, C.c_float, PF]),
     ("fem_solver_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_batch_create", C.c_int, [C.POINTER(P), I32, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_batch_destroy", C.c_int, [P]),

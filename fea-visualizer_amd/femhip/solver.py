@@ -211,6 +211,16 @@ class FEMContext:
         self.last_solve_stats = st
         return x
 
+    def smooth_normals(self, normals, values, extrusion: float):
+        """Summed (unnormalised) normals of the surface extruded by `values`
+        (smooth_normals.glsl:22-57), computed on the GPU."""
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        val = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        out = np.empty_like(nrm)
+        L.check(self._lib.fem_ctx_smooth_normals(self.handle, L.ptr(nrm, C.c_float), L.ptr(val, C.c_float),
+                                                 float(extrusion), L.ptr(out, C.c_float)))
+        return out
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.fem_ctx_destroy(self.handle)
@@ -299,6 +309,15 @@ class HIPSolver:
 
     def sync(self):
         L.check(self._lib.fem_solver_sync(self.handle))
+
+    def smooth_normals(self, normals, extrusion: float):
+        """Normals of the surface extruded by the current field, from the
+        device state (Surface::calculate_normals after advance_time)."""
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(nrm)
+        L.check(self._lib.fem_solver_smooth_normals(self.handle, L.ptr(nrm, C.c_float), float(extrusion),
+                                                    L.ptr(out, C.c_float)))
+        return out
 
     def init_newmark_acceleration(self):
         """Consistent initial acceleration of the Newmark wave from the current

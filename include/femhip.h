@@ -181,6 +181,16 @@ int fem_ctx_get_matrix(const fem_ctx* ctx, int32_t which, double* values);
  * opts->warm_start, else x0 = 0 as Eigen's solve(). */
 int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                   fem_step_stats* stats);
+/* Smooth vertex normals of the surface extruded by per-vertex values
+ * (shaders/FEM/smooth_normals.glsl:22-57): every vertex moves along its
+ * normal by min(1, extrusion * max(0, value)); each triangle's unnormalised
+ * normal cross(b - a, c - a) of the moved positions is summed into its three
+ * vertices.  Each vertex sums its incident triangles in ascending order, so
+ * the result is deterministic (the shader accumulates with float
+ * compare-and-swap in arbitrary order).  Host arrays: normals and out
+ * [nv][3], values [nv]. */
+int fem_ctx_smooth_normals(fem_ctx* ctx, const float* normals_nv3, const float* values_nv, float extrusion,
+                           float* out_nv3);
 
 /* ---- solver: the Solver interface (Solver.hpp:8-15) ----
  * fem_solver_advance  <- Solver::advance_time        (CPUSolver.cpp:64-147)
@@ -203,6 +213,13 @@ int fem_solver_unstable(fem_solver* s, int32_t* unstable);
 int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr);
 /* Synchronise the solver stream. */
 int fem_solver_sync(fem_solver* s);
+/* Smooth normals of the surface extruded by the solver's current field (the
+ * visualisation step after advance_time: Surface::calculate_normals,
+ * src/Utils/Surface.cpp:287-307, and shaders/FEM/smooth_normals.glsl:22-57),
+ * with the values taken from the state on the device: u, or v for Gray-Scott,
+ * boundary vertices 0.  normals_nv3: the mesh's vertex normals [nv][3];
+ * out_nv3 receives the unnormalised sums [nv][3]. */
+int fem_solver_smooth_normals(fem_solver* s, const float* normals_nv3, float extrusion, float* out_nv3);
 /* Autotuner (no reference counterpart): runs probe_steps under a few launch
  * plans from the current state -- persistent team sizes in both exchange
  * modes, and the streaming engine -- restores the state after each, and keeps

@@ -16,6 +16,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   wave, grayscott)        heat mass conservation under Neumann; wave energy decay
   newmark                 discrete energy conservation (beta=1/4, gamma=1/2)
   adjoint                 central finite differences and a dot-product test
+  smooth_normals          closed form on flat meshes (2 x incident area along z),
+                          outward normals on the sphere
   port_run_f              same steps as orc_step_f (FP32) up to CG tolerance
 """
 from __future__ import annotations
@@ -194,6 +196,26 @@ class System:
                                  dt, nsteps, _p(np.ascontiguousarray(u0, dtype=np.float64)),
                                  _p(np.ascontiguousarray(target, dtype=np.float64)), tol, _p(uT), _p(gu), _p(gk))
         return loss, uT, gu, gk
+
+
+def smooth_normals(xyz, tri, normals, values, extrusion):
+    """shaders/FEM/smooth_normals.glsl:22-57 in float32: positions moved along
+    their normals by min(1, extrusion max(0, value)), each triangle's
+    unnormalised normal cross(b - a, c - a) summed into its vertices (here in
+    triangle order; the shader's float compare-and-swap order is arbitrary)."""
+    f = np.float32
+    x = np.asarray(xyz, dtype=f)
+    nrm = np.asarray(normals, dtype=f).reshape(-1, 3)
+    v = np.asarray(values, dtype=f)
+    h = np.minimum(f(1.0), f(extrusion) * np.maximum(f(0.0), v))
+    p = x + h[:, None] * nrm
+    t = np.asarray(tri, dtype=np.int64)
+    a, b, c = p[t[:, 0]], p[t[:, 1]], p[t[:, 2]]
+    n = np.cross(b - a, c - a).astype(f)
+    out = np.zeros_like(x)
+    for k in range(3):
+        np.add.at(out, t[:, k], n)
+    return out
 
 
 def csr_to_csc(n, rp, ci, vals):

@@ -217,3 +217,24 @@ def test_newmark_initial_acceleration(gpu):
     a = sol.init_newmark_acceleration()
     assert rel(a, a_ref) <= 1e-10
     assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_A), a_ref) <= 1e-10
+
+
+def test_smooth_normals_vs_oracle(gpu):
+    from femhip import Equation, HIPSolver, ReactionDiffusionParameters
+    m = meshes.icosphere(4)
+    p = m["xyz"].astype(np.float32)
+    nrm = p / np.linalg.norm(p, axis=1, keepdims=True)
+    rng = np.random.Generator(np.random.PCG64(21))
+    vals = rng.uniform(-0.5, 1.5, p.shape[0]).astype(np.float32)
+    ctx = FEMContext(m, 1)
+    got = ctx.smooth_normals(nrm, vals, 0.3)
+    ref = oracle.smooth_normals(p, m["tri"], nrm, vals, 0.3)
+    scale = np.linalg.norm(ref, axis=1).max()
+    assert np.abs(got - ref).max() <= 1e-5 * scale
+    # from the solver state (Gray-Scott: the surface carries v)
+    ctx.equation = Equation.Reaction_Diffusion
+    ctx.parameters[Equation.Reaction_Diffusion] = ReactionDiffusionParameters()
+    sol = HIPSolver(ctx)
+    sol.set_state(_lib.FEM_FIELD_V, vals)
+    got2 = sol.smooth_normals(nrm, 0.3)
+    assert np.array_equal(got2, got)

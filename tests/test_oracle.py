@@ -322,3 +322,69 @@ def test_shipped_tube_counts():
     assert oracle.System(m, bc=1).nnz == 32544
     idx, rp, ci = brute_pattern(m, 0)
     assert np.array_equal(s.row_ptr, rp) and np.array_equal(s.col_idx, ci)
+
+
+def test_smooth_normals_oracle_closed_forms():
+    """Flat mesh, zero values: each vertex sums 2 x (area of its incident
+    triangles) along +z (counter-clockwise triangles).  Icosphere with zero
+    values: every summed normal points outward.  Extrusion clamps at 1."""
+    m = meshes.jittered_square(9, seed=3)
+    x = m["xyz"].astype(np.float64)
+    t = m["tri"].astype(np.int64)
+    nrm = np.tile([0.0, 0.0, 1.0], (x.shape[0], 1))
+    out = oracle.smooth_normals(x, t, nrm, np.zeros(x.shape[0]), 5.0)
+    e1, e2 = x[t[:, 1]] - x[t[:, 0]], x[t[:, 2]] - x[t[:, 0]]
+    area2 = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    expect = np.zeros(x.shape[0])
+    for k in range(3):
+        np.add.at(expect, t[:, k], area2)
+    assert np.all(area2 > 0)
+    assert np.allclose(out[:, 2], expect, rtol=1e-5, atol=1e-7) and np.allclose(out[:, :2], 0.0, atol=1e-7)
+    # uniform extrusion of a flat mesh along +z is a translation: same normals
+    moved = oracle.smooth_normals(x, t, nrm, np.full(x.shape[0], 0.3), 2.0)
+    assert np.allclose(moved, out, rtol=1e-5, atol=1e-7)
+    # values beyond 1/extrusion clamp to a unit move: same as value 1/extrusion
+    big = oracle.smooth_normals(x, t, nrm, np.full(x.shape[0], 7.0), 2.0)
+    assert np.allclose(big, out, rtol=1e-5, atol=1e-7)
+    s = meshes.icosphere(2)
+    p = s["xyz"].astype(np.float64)
+    n_out = oracle.smooth_normals(p, s["tri"], p / np.linalg.norm(p, axis=1, keepdims=True), np.zeros(p.shape[0]), 1.0)
+    assert np.all((n_out * p).sum(axis=1) > 0)
+
+
+def test_ply_export_clipping_cases(tmp_path):
+    """Surface::export_to_ply's three clipping cases on one triangle, the
+    closed and mirrored caps, de-duplication and the file round trip."""
+    from femhip import export
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    nrm = np.tile(np.float32([0, 0, 1]), (3, 1))
+    tri = np.array([[0, 1, 2]])
+    # all above: one triangle, positions extruded by value * extrusion
+    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 2.0, 3.0], 0.5, 0.0)
+    assert f.tolist() == [[0, 1, 2]] and np.allclose(pos[:, 2], [0.5, 1.0, 1.5])
+    # one above (vertex 0 at 1, others 0, threshold 0.5): the cut points sit half way
+    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 0.0, 0.0], 1.0, 0.5)
+    assert len(f) == 1 and len(pos) == 3
+    assert np.allclose(sorted(map(tuple, pos.tolist())), sorted([(0, 0, 1), (0.5, 0, 0.5), (0, 0.5, 0.5)]))
+    assert np.allclose(sorted(val.tolist()), [0.5, 0.5, 1.0])
+    # two above: a quad split in two triangles; CLOSED adds the projected cap (2 more faces)
+    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5)
+    assert len(f) == 2 and len(pos) == 4
+    pos_c, _, f_c = export.clip_surface(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5, export.CLOSED)
+    assert len(f_c) == 4 and len(pos_c) == 6
+    # none above: nothing
+    pos, val, f = export.clip_surface(xyz, tri, nrm, [0.0, 0.0, 0.0], 1.0, 0.5)
+    assert len(f) == 0 and len(pos) == 0
+    # shared vertices are written once; file round trip and colours
+    m = meshes.structured_square(4)
+    n = np.tile(np.float32([0, 0, 1]), (m["xyz"].shape[0], 1))
+    v = np.linspace(0.0, 1.0, m["xyz"].shape[0])
+    path = str(tmp_path / "s.ply")
+    pos, val, f = export.export_ply(path, m["xyz"], m["tri"], n, v, 0.2, -1.0)
+    assert len(pos) == m["xyz"].shape[0] and len(f) == m["tri"].shape[0]
+    p2, col, f2 = export.read_ply(path)
+    assert np.allclose(p2, pos, rtol=1e-6, atol=1e-7) and np.array_equal(f2, f)
+    cm = export.polynomial_color_map(export.VIRIDIS)
+    assert np.allclose(col[0], cm(val[0]), atol=1e-6)
+    # viridis endpoints: dark purple at 0, yellow at 1
+    assert cm(0.0)[2] > cm(0.0)[1] and cm(1.0)[0] > 0.9 and cm(1.0)[1] > 0.85
