@@ -592,6 +592,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "spmv":
+        import torch  # noqa: F401  (before the library, as above)
     if args.workload != "grayscott":
         fn = {"heat": run_heat_c1, "wave": run_wave_c3, "batched": run_batched_c4, "spmv": run_spmv,
               "adjoint": run_adjoint}[args.workload]
@@ -600,6 +602,10 @@ def main():
     if args.steps is None:
         args.steps = 2000
 
+    if not args.no_spmv:
+        # torch (device buffers of the SpMV leg) must load its HIP runtime
+        # before the library does: a process holds one runtime
+        import torch  # noqa: F401
     from femhip import Equation, FEMContext, HIPSolver, ReactionDiffusionParameters, SolverOptions, _lib, meshes
 
     mesh = meshes.icosphere(args.level)
@@ -620,6 +626,7 @@ def main():
         s.set_state_unknowns(_lib.FEM_FIELD_V, v0)
         return s
 
+    log("grayscott: N =", n)
     sol = solver()
     tuned = None
     if args.team_size == 0 and not args.no_autotune:
@@ -633,6 +640,7 @@ def main():
     status = sol.advance(args.steps)
     sol.sync()
     wall = time.perf_counter() - t0
+    log("grayscott: timed run", wall)
     st = sol.last_stats
     kernel_ms = st.ms
     dof_steps = 2 * n * args.steps
@@ -718,11 +726,14 @@ def main():
         pl.close()
 
     if not args.no_spmv:
+        log("spmv")
         out["spmv"] = spmv_measure()
     if not args.no_adjoint:
+        log("adjoint")
         out["adjoint"] = adjoint_measure()
 
     if not args.no_cpu and args.cpu_steps > 0:
+        log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(mesh, u0, v0, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], args.cpu_steps)
     print(json.dumps(out))
 

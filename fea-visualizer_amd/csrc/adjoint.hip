@@ -96,7 +96,7 @@ int adjoint_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_adjoint** out)
   fem_solver_opts o;
   fem_opts_default(&o);
   if (opts) o = *opts;
-  if (o.engine == FEM_ENGINE_PLAIN) {
+  if (o.engine == FEM_ENGINE_PLAIN || o.engine == FEM_ENGINE_GL_COMPAT) {
     set_error("the adjoint runs on the persistent or streaming engine");
     return FEM_ERR_INVALID;
   }

@@ -410,7 +410,7 @@ int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opt
       set_error("fem_ctx_solve: null argument");
       return FEM_ERR_INVALID;
     }
-    if (opts && opts->engine == FEM_ENGINE_PLAIN) {
+    if (opts && (opts->engine == FEM_ENGINE_PLAIN || opts->engine == FEM_ENGINE_GL_COMPAT)) {
       set_error("fem_ctx_solve runs on the persistent or streaming engine");
       return FEM_ERR_INVALID;
     }

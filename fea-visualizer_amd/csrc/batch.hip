@@ -462,7 +462,7 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
     return FEM_ERR_INVALID;
   }
   if (b->opts.engine != FEM_ENGINE_PERSISTENT && b->opts.engine != FEM_ENGINE_PLAIN &&
-      b->opts.engine != FEM_ENGINE_STREAM) {
+      b->opts.engine != FEM_ENGINE_STREAM && b->opts.engine != FEM_ENGINE_GL_COMPAT) {
     delete b;
     set_error("unknown engine");
     return FEM_ERR_INVALID;
@@ -805,6 +805,7 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   }
   if (b->opts.engine == FEM_ENGINE_PLAIN) return plain_advance(b, nsteps, stats);
   if (b->opts.engine == FEM_ENGINE_STREAM) return stream_advance(b, nsteps, stats);
+  if (b->opts.engine == FEM_ENGINE_GL_COMPAT) return gl_advance(b, nsteps, stats);
   int rc = batch_prepare(b, true);
   if (rc) return rc;
   const int ninst = (int)b->insts.size();

@@ -116,6 +116,7 @@ int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
 int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_surface_values(fem_batch* b, int inst, float* d_out);
+int gl_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_set_linear(fem_batch* b, int inst, const double coef[3]);
 int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                  fem_step_stats* stats);

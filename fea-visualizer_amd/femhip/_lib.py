@@ -42,6 +42,7 @@ FEM_MATRIX_A = 2
 FEM_ENGINE_PERSISTENT = 0
 FEM_ENGINE_PLAIN = 1
 FEM_ENGINE_STREAM = 2
+FEM_ENGINE_GL_COMPAT = 3
 FEM_PREC_FP64 = 0
 FEM_PREC_FP32_IR = 1
 FEM_KRYLOV_AUTO = 0

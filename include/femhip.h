@@ -63,7 +63,12 @@ enum { FEM_MATRIX_K = 0, FEM_MATRIX_M = 1, FEM_MATRIX_A = 2 };
  * PLAIN: FP64 baseline, one right-hand side at a time with host-read scalars;
  * STREAM: each phase a full-occupancy launch over the row blocks of all
  * instances (bandwidth-bound instances: large meshes, many right-hand sides). */
-enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1, FEM_ENGINE_STREAM = 2 };
+enum { FEM_ENGINE_PERSISTENT = 0, FEM_ENGINE_PLAIN = 1, FEM_ENGINE_STREAM = 2, FEM_ENGINE_GL_COMPAT = 3 };
+/* GL_COMPAT reproduces the reference's OpenGL compute-shader solver for A/B
+ * comparison (src/FEM/GPUSolver.cpp:74-172): FP32, warm start, unpreconditioned
+ * CG with at most max_iters (default 10) iterations, Gray-Scott reaction summed
+ * per row slot and the v solve after the u solve, instability when the last
+ * r.r > 1e4.  Not the accurate path. */
 enum { FEM_PREC_FP64 = 0, FEM_PREC_FP32_IR = 1 };
 /* FEM_KRYLOV_CG is single-reduction (Chronopoulos-Gear) PCG;
  * FEM_KRYLOV_CG_CLASSIC the two-reduction form of Eigen's conjugate_gradient. */

@@ -16,6 +16,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   wave, grayscott)        heat mass conservation under Neumann; wave energy decay
   newmark                 discrete energy conservation (beta=1/4, gamma=1/2)
   adjoint                 central finite differences and a dot-product test
+  gl_step                 its CG reaches the direct solution given enough
+                          iterations (heat), and it follows the GL stopping rule
   smooth_normals          closed form on flat meshes (2 x incident area along z),
                           outward normals on the sphere
   port_run_f              same steps as orc_step_f (FP32) up to CG tolerance
@@ -196,6 +198,74 @@ class System:
                                  dt, nsteps, _p(np.ascontiguousarray(u0, dtype=np.float64)),
                                  _p(np.ascontiguousarray(target, dtype=np.float64)), tol, _p(uT), _p(gu), _p(gk))
         return loss, uT, gu, gk
+
+
+def gl_step(sysf, equation, prm, u, v, max_it=10):
+    """One frame of the reference's OpenGL solver (GPUSolver.cpp:74-172 with
+    cgm_helper.glsl:74-117 and cgm.glsl:75-148) in float32: warm start,
+    unpreconditioned CG for at most max_it iterations while r.r > 1e-10,
+    stopping once r.r <= 1e-10 r0.r0; Gray-Scott reaction summed over the
+    row's slots with the neighbours' values, u solved first and the v solve
+    using the new u.  Returns (u, v, last r.r).  (The shader's reductions and
+    direction update race; this restatement uses their intended values.)"""
+    f = np.float32
+    n, rp, ci = sysf.n, sysf.row_ptr, sysf.col_idx
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    Mf, Kf, Af = (np.asarray(x, dtype=f) for x in (sysf.M, sysf.K, sysf.A))
+    dt = f(prm[0])
+
+    def rowsum(vals):
+        return np.bincount(rows, weights=np.asarray(vals, dtype=np.float64), minlength=n).astype(f)
+
+    def coef(eq):
+        if eq == HEAT:
+            return Mf / dt + f(prm[1]) * Kf
+        if eq == ADVDIFF:
+            return Mf / dt + f(prm[1]) * Kf - Af
+        if eq == WAVE:
+            return Mf / dt + f(prm[1]) * f(prm[1]) * Kf * dt
+        if eq == "gs_u":
+            return Mf / dt + f(prm[1]) * Kf
+        return Mf / dt + f(prm[2]) * Kf
+
+    def cgm(S, b, x):
+        r = (b - rowsum(S * x[ci])).astype(f)
+        d = r.copy()
+        ri = f(np.dot(r.astype(np.float64), r))
+        r0 = ri
+        it = 0
+        while ri > f(1e-10) and it < max_it:
+            ad = rowsum(S * d[ci])
+            dad = f(np.dot(d.astype(np.float64), ad))
+            alpha = f(ri / dad)
+            r = (r - alpha * ad).astype(f)
+            x = (x + alpha * d).astype(f)
+            ri1 = f(np.dot(r.astype(np.float64), r))
+            d = (r + f(ri1 / ri) * d).astype(f)
+            ri = ri1
+            it += 1
+            if ri <= f(1e-10) * r0:
+                break
+        return x, ri
+
+    u = np.asarray(u, dtype=f).copy()
+    v = np.asarray(v, dtype=f).copy()
+    if equation == GRAYSCOTT:
+        F, kr = f(prm[3]), f(prm[4])
+        uc, vc = u[ci], v[ci]
+        b = rowsum(uc * (Mf / dt) - uc * vc * vc + F * (f(1) - uc))
+        u, rr = cgm(coef("gs_u"), b, u)
+        uc = u[ci]
+        b = rowsum(vc * (Mf / dt) + uc * vc * vc - vc * (F + kr))
+        v, rr = cgm(coef("gs_v"), b, v)
+        return u, v, rr
+    if equation == WAVE:
+        b = rowsum(v[ci] * (Mf / dt) - f(prm[1]) * f(prm[1]) * u[ci] * Kf)
+        v, rr = cgm(coef(WAVE), b, v)
+        return (u + v * dt).astype(f), v, rr
+    b = rowsum(u[ci] * (Mf / dt))
+    u, rr = cgm(coef(equation), b, u)
+    return u, v, rr
 
 
 def smooth_normals(xyz, tri, normals, values, extrusion):

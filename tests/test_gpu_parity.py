@@ -434,3 +434,47 @@ def test_stream_engine_adjoint_matches_persistent(gpu):
         adj.close()
     for a_, b_ in zip(out[0], out[1]):
         assert rel(b_, a_) < 1e-10
+
+
+@pytest.mark.parametrize("eq", ["heat", "advdiff", "wave", "grayscott"])
+def test_gl_compat_engine_vs_oracle(gpu, eq):
+    """The GL-compatible engine against the float32 restatement of the
+    reference's OpenGL solver (10 CG iterations per solve, warm start)."""
+    print("case gl", eq, flush=True)
+    if eq == "grayscott":
+        m = meshes.icosphere(3)
+        s = oracle.System(m, bc=1)
+        rng = np.random.default_rng(3)
+        u = np.zeros(s.n)
+        v = np.where(rng.uniform(size=s.n) < 0.1, 0.25, 0.0)
+        prm = ReactionDiffusionParameters()  # reference defaults
+        ctx = FEMContext(m, 1)
+        sol = make_solver(ctx, Equation.Reaction_Diffusion, prm, engine=_lib.FEM_ENGINE_GL_COMPAT)
+        sol.set_state_unknowns(_lib.FEM_FIELD_V, v)
+        op = [prm.time_step, prm.Du, prm.Dv, prm.feed_rate, prm.kill_rate]
+        code = oracle.GRAYSCOTT
+    else:
+        m = meshes.jittered_square(30, seed=5)
+        s = oracle.System(m, bc=0)
+        u = s.gather(gauss(m, (0.45, 0.5), 0.02))
+        v = np.zeros(s.n)
+        ctx = FEMContext(m, 0)
+        if eq == "heat":
+            sol = make_solver(ctx, Equation.Heat, HeatParameters(), engine=_lib.FEM_ENGINE_GL_COMPAT)
+            op, code = [0.01, 0.05], oracle.HEAT
+        elif eq == "advdiff":
+            sol = make_solver(ctx, Equation.Advection_Diffusion, AdvectionDiffusionParameters(),
+                              engine=_lib.FEM_ENGINE_GL_COMPAT)
+            op, code = [0.001, 0.25], oracle.ADVDIFF
+        else:
+            sol = make_solver(ctx, Equation.Wave, WaveParameters(), engine=_lib.FEM_ENGINE_GL_COMPAT)
+            op, code = [0.05, 0.05], oracle.WAVE
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    for _ in range(4):
+        u, v, rr = oracle.gl_step(s, code, op, u, v)
+    assert sol.advance_time(4) == _lib.FEM_OK
+    if eq == "grayscott":
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < 1e-4
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < 1e-4
+    else:
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < 1e-4

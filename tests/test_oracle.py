@@ -388,3 +388,31 @@ def test_ply_export_clipping_cases(tmp_path):
     assert np.allclose(col[0], cm(val[0]), atol=1e-6)
     # viridis endpoints: dark purple at 0, yellow at 1
     assert cm(0.0)[2] > cm(0.0)[1] and cm(1.0)[0] > 0.9 and cm(1.0)[1] > 0.85
+
+
+def test_gl_step_oracle_pins():
+    """The GL-semantics restatement: with enough iterations its warm-started,
+    unpreconditioned CG reaches the direct solution of the same system; with
+    the reference's 10 iterations it stops early on a fine mesh; its
+    Gray-Scott RHS sums the reaction over the row's slots."""
+    import scipy.sparse.linalg as sla
+    m = meshes.jittered_square(12, seed=2)
+    s = oracle.System(m, bc=0)
+    u0 = s.gather(np.exp(-((m["xyz"][:, 0] - 0.5) ** 2 + (m["xyz"][:, 1] - 0.5) ** 2) / 0.02))
+    dt, c = 1e-2, 1.0
+    S = (s.scipy("M") / dt + c * s.scipy("K")).tocsc()
+    exact = sla.spsolve(S, s.scipy("M") @ u0 / dt)
+    u, _, rr = oracle.gl_step(s, oracle.HEAT, [dt, c], u0, np.zeros(s.n), max_it=500)
+    assert np.linalg.norm(u - exact) <= 1e-5 * np.linalg.norm(exact)
+    u10, _, rr10 = oracle.gl_step(s, oracle.HEAT, [dt, c], u0, np.zeros(s.n), max_it=2)
+    assert rr10 > rr and np.linalg.norm(u10 - exact) > np.linalg.norm(u - exact)
+    # Gray-Scott from u = v = 0: the u right-hand side is F x (slots of the
+    # row), and one CG step from x = 0 is alpha b with alpha = b.b / b.S b;
+    # the v system then has a zero right-hand side
+    F = 0.035
+    gu, gv, _ = oracle.gl_step(s, oracle.GRAYSCOTT, [1e-3, 0.08, 0.04, F, 0.06], np.zeros(s.n), np.zeros(s.n),
+                               max_it=1)
+    b = F * np.diff(s.row_ptr).astype(np.float64)
+    Su = (s.scipy("M") / 1e-3 + 0.08 * s.scipy("K"))
+    alpha = (b @ b) / (b @ (Su @ b))
+    assert np.allclose(gu, alpha * b, rtol=1e-5) and not np.any(gv)
