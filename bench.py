@@ -416,8 +416,12 @@ def run_batched_c4(args):
             b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
         return b
     log("batched: contexts assembled, N =", tot_n)
-    b = make(tol=1e-10)
-    tuned = b.autotune(2) if not args.no_autotune else None
+    if args.engine == "stream":
+        b = make(tol=1e-10, engine=_lib.FEM_ENGINE_STREAM)
+        tuned = None
+    else:
+        b = make(tol=1e-10)
+        tuned = b.autotune(2) if not args.no_autotune else None
     log("batched: autotuned", tuned)
     engine = "stream" if b.options.engine == _lib.FEM_ENGINE_STREAM else "persistent"
     for i in range(len(ctxs)):
@@ -439,7 +443,8 @@ def run_batched_c4(args):
                         {"workload": f"batched advection-diffusion, {len(ctxs)} jittered meshes of side "
                                      f"log-U[32,512] (sum N={tot_n}, sum nnz={tot_nnz}), 16 RHS each, dt=1e-3, "
                                      f"tol 1e-10", "global_batch": len(ctxs) * 16,
-                         "parallelism": f"single GPU, all instances per launch ({engine} engine, autotuned)"},
+                         "parallelism": f"single GPU, all instances per launch ({engine} engine"
+                                         f"{', autotuned' if tuned is not None else ''})"},
                         status=int(status), all_finite=bool(finite),
                         solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
                                 "kernel_ms": st.ms},
@@ -572,6 +577,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
+                    help="batched: 'stream' runs the streaming engine without autotuning")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-spmv", action="store_true")

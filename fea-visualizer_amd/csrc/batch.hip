@@ -49,6 +49,14 @@ __global__ void k_form(int n, const int* rp, const int* diag, const int* tperm, 
   }
 }
 
+// Column-scaled inner matrix S_ij D^-1_jj (one value column).
+template <typename TI>
+__global__ void k_colscale(int n, const int* rp, const int* ci, const TI* Si, const TI* dinvi, TI* SDi) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int s = rp[r]; s < rp[r + 1]; s++) SDi[s] = Si[s] * dinvi[ci[s]];
+}
+
 // Barrier epochs restart at 1 every launch.  Barrier e uses buffer e & 3 and
 // tag (e >> 2) & 1 in the lowest mantissa bit, so before a launch buffer 0
 // must carry tag 0 and buffers 1-3 tag 1 (none of them looks "arrived").
@@ -89,7 +97,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 }
 
 void free_inst(InstHost& h) {
-  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_u, h.d_v, h.d_a, h.d_x, h.d_b, h.d_vec,
+  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v, h.d_a, h.d_x, h.d_b, h.d_vec,
                   h.d_status, h.d_counters, h.d_last_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -113,6 +121,7 @@ int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
   h.d_Si = (void*)dalloc<char>((size_t)nnz * kv * tb, &rc);
   h.d_dinv64 = dalloc<double>((size_t)n * kv, &rc);
   h.d_dinvi = (void*)dalloc<char>((size_t)n * kv * tb, &rc);
+  if (kv == 1 && k >= 4) h.d_SDi = (void*)dalloc<char>((size_t)nnz * tb, &rc);
   h.d_u = dalloc<double>((size_t)n * k, &rc);
   h.d_v = dalloc<double>((size_t)n * k, &rc);
   h.d_a = dalloc<double>((size_t)n * k, &rc);
@@ -186,6 +195,15 @@ int form_inst(fem_batch* b, InstHost& h) {
                          c->d_diag, tp, c->d_M, c->d_K, c->d_A, h.kv, cf, h.d_S64, (double*)h.d_Si, h.d_dinv64,
                          (double*)h.d_dinvi);
     FEM_HIP(hipGetLastError());
+    if (h.d_SDi) {
+      if (inner_bytes(b) == 4)
+        hipLaunchKernelGGL(k_colscale<float>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n, c->d_rp,
+                           c->d_ci, (const float*)h.d_Si, (const float*)h.d_dinvi, (float*)h.d_SDi);
+      else
+        hipLaunchKernelGGL(k_colscale<double>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, b->stream, c->n,
+                           c->d_rp, c->d_ci, (const double*)h.d_Si, (const double*)h.d_dinvi, (double*)h.d_SDi);
+      FEM_HIP(hipGetLastError());
+    }
   }
   h.formed_version = c->version;
   h.dirty = false;
@@ -229,6 +247,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.Si = h.d_Si;
   I.dinv64 = h.d_dinv64;
   I.dinvi = h.d_dinvi;
+  I.SDi = h.d_SDi;
   I.lumped = c->d_lumped;
   I.u = h.d_u;
   I.v = h.d_v;
@@ -842,11 +861,16 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   if (rc) return rc;
   const int ninst = (int)b->insts.size();
   const int k = b->insts[0].k, kr = b->insts[0].dev.krylov;
-  for (auto& h : b->insts)
+  for (auto& h : b->insts) {
     if (h.k != k || h.dev.krylov != kr) {
       set_error("batch instances must share the number of columns and the Krylov method");
       return FEM_ERR_INVALID;
     }
+    if (k >= 4 && (h.kv != 1 || !h.d_SDi)) {  // the 16-byte column layout shares one matrix across columns
+      set_error("stream engine: instances of four or more columns need one matrix for all columns");
+      return FEM_ERR_INVALID;
+    }
+  }
   const int fp32 = inner_bytes(b) == 4;
   if (b->sinsts_n != ninst) {
     if (b->d_sinsts) (void)hipFree(b->d_sinsts);
@@ -857,7 +881,8 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   std::vector<Inst> hi(ninst);
   for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
   FEM_HIP(hipMemcpyAsync(b->d_sinsts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
-  const int rows = b->stream_rows > 0 ? b->stream_rows : (k >= 16 ? 64 : 256);
+  // rows per row block: team_size when given (the stream engine has no teams)
+  const int rows = b->opts.team_size > 0 ? b->opts.team_size : b->stream_rows > 0 ? b->stream_rows : (k >= 16 ? 128 : 256);
   if (!b->splan || b->splan_key != (long long)rows * 1000003LL + k * 131 + kr * 7 + fp32) {
     if (b->splan) stream_plan_free(b->splan);
     b->splan = nullptr;
@@ -951,19 +976,31 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
       if (rc) break;
     }
   }
-  // the streaming engine as one more candidate
+  // the streaming engine as more candidates: rows per row block
   bool stream_best = false;
+  const int saved_rows = b->stream_rows;
+  int best_rows = saved_rows;
   if (!rc) {
     b->opts.engine = FEM_ENGINE_STREAM;
-    if (!(rc = restore())) {
+    const int team = b->opts.team_size;
+    b->opts.team_size = 0;  // rows come from stream_rows
+    const int k = b->launch.k;
+    for (int rows : {k >= 16 ? 64 : 128, k >= 16 ? 128 : 256, k >= 16 ? 256 : 512}) {
+      b->stream_rows = rows;
+      if ((rc = restore())) break;
       fem_step_stats st{};
       const int r = batch_advance(b, probe, &st);
-      if (r < 0) rc = r;
-      else if (st.ms < best) {
+      if (r < 0) {
+        rc = r;
+        break;
+      }
+      if (st.ms < best) {
         best = st.ms;
         stream_best = true;
+        best_rows = rows;
       }
     }
+    b->opts.team_size = team;
     b->opts.engine = FEM_ENGINE_PERSISTENT;
   }
   const int rr = restore();
@@ -973,7 +1010,11 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   if (single) b->opts.team_size = rc ? saved_team : best_c;
   else b->grid_budget = rc ? saved_budget : best_c;
   b->mode_override = rc ? saved_mode : best_m;
-  if (!rc && stream_best) b->opts.engine = FEM_ENGINE_STREAM;
+  b->stream_rows = (!rc && stream_best) ? best_rows : saved_rows;
+  if (!rc && stream_best) {
+    b->opts.engine = FEM_ENGINE_STREAM;
+    b->opts.team_size = 0;
+  }
   b->plan_ready = false;
   if (chosen) *chosen = (!rc && stream_best) ? 0 : best_c;
   if (best_ms) *best_ms = best;

@@ -67,6 +67,8 @@ struct Inst {
   const void* Si;        // [nnz][kv] inner precision (float or double)
   const double* dinv64;  // [n][kv]
   const void* dinvi;     // [n][kv] inner precision
+  const void* SDi;       // [nnz] inner precision S_ij D^-1_jj (one value column, four or more
+                         //   right-hand sides; null otherwise)
   const double* lumped;  // [n]
   double* u;             // state [n][k]  (Gray-Scott: u,v interleaved, k = 2)
   double* v;

@@ -53,6 +53,7 @@ struct InstHost {
   void* d_Si = nullptr;
   double* d_dinv64 = nullptr;
   void* d_dinvi = nullptr;
+  void* d_SDi = nullptr;     // column-scaled inner matrix (Inst::SDi)
   double* d_u = nullptr;
   double* d_v = nullptr;
   double* d_a = nullptr;

@@ -16,9 +16,11 @@
 // solve has finished skip the launches.  The host reads one "all finished"
 // flag per iteration to stop the loop.
 //
-// The phase bodies are the persistent kernel's own (plain exchange mode), so
-// both engines compute the same arithmetic.
-#include "step_kernel.h"
+// For one or two right-hand sides the phase bodies are the persistent
+// kernel's own (plain exchange mode); for four or more, each lane owns 16
+// bytes of adjacent columns (stream_vec.h).  Both compute the persistent
+// kernel's per-element arithmetic.
+#include "stream_vec.h"
 
 #include <algorithm>
 
@@ -52,44 +54,46 @@ struct SArgs {
   SState* ss;
   const int* blk_inst;  // [nblocks]
   const int* blk_row;   // [nblocks] first row
-  int rows;             // rows per block
   int par;              // buffer parity of the current iteration
   int first;            // first iteration of the sweep
   int step;             // step index inside the launch sequence (trajectory records)
   int pass;             // FP64 residual: 0 normal, 1 redo with the correction dropped
   double* partials;     // [nblocks][3 K]
+  const int* act_blk;   // launched block -> row block (null: every row block, in order)
+  const int* act_inst;  // reduction block -> instance (null: every instance)
+  const int* blk_end;   // [nblocks] end row
+  int stage_rows;       // LDS stage capacity: rows and nonzeros per row block
+  int stage_nnz;
 };
 
+// Nonzeros a row block may hold: blocks are cut so that their pattern fits
+// the LDS stage; a single longer row gets a block of its own and the stage
+// grows to it (up to the workgroup's 64 KiB of LDS).
+constexpr int kStageCap = 2048;
+constexpr size_t kMaxLds = 65536 - sizeof(Shared);
+
+template <int PH>
+constexpr bool staged_phase() {
+  return PH == PH_PCG_INIT || PH == PH_PCG_ITER || PH == PH_CG_A || PH == PH_BI_P || PH == PH_BI_S;
+}
+
+// Launched block -> position in the block list, so that each XCD (workgroups
+// are dealt to the 8 XCDs round-robin) walks one contiguous range of row
+// blocks: the halo rows of a block are then mostly rows its own XCD's L2
+// already holds.  A bijection on [0, n).
+__device__ inline int xcd_chunk(int b, int n) {
+  const int x = b & 7, q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
+// One column per lane (the persistent kernel's phase bodies).
 template <int K, typename TI, int PH>
-__global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
-  __shared__ Shared sh;
-  const int b = blockIdx.x;
-  const int i = a.blk_inst[b];
-  const Inst& I = a.insts[i];
-  const SState& S = a.ss[i];
-  // skip rules (uniform per block)
-  if constexpr (PH == PH_RES) {
-    if (a.pass ? !S.redo : S.solve_done) return;
-  } else if constexpr (PH != PH_RHS && PH != PH_UPD) {
-    if (S.sweep_done) return;
-  }
-  if (threadIdx.x < K) {
-    const int j = threadIdx.x;
-    sh.gamma[j] = S.gamma[j];
-    sh.alpha[j] = S.alpha[j];
-    sh.beta[j] = S.beta[j];
-    sh.omega[j] = S.omega[j];
-    sh.bb[j] = S.bb[j];
-    sh.reset[j] = (PH == PH_RES && a.pass) ? S.reset[j] : 0;
-  }
-  __syncthreads();
-  const int row_lo = a.blk_row[b];
-  const int row_hi = min(I.n, row_lo + a.rows);
+__device__ void run_scalar(const Inst& I, const SArgs& a, Shared& sh, int b, int row_lo, int row_hi) {
   constexpr int NS = nsums<PH>();
-  double acc[NS > 0 ? NS : 1];
-  for (int q = 0; q < (NS > 0 ? NS : 1); q++) acc[q] = 0.0;
   const bool first = a.first != 0;
   const int par = a.par;
+  double acc[NS > 0 ? NS : 1];
+  for (int q = 0; q < (NS > 0 ? NS : 1); q++) acc[q] = 0.0;
   TI* r0 = (TI*)I.r;
   if constexpr (PH == PH_RHS) {
     phase_rhs<K, TI, kPlain>(I, row_lo, row_hi, r0, a.step, acc);
@@ -112,15 +116,14 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   } else if constexpr (PH == PH_BI_P) {
     TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
     TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
-    phase_bi_p<K, TI, kPlain>(I, row_lo, row_hi, (TI*)I.r, pb[par], pb[par ^ 1], vb[par], vb[par ^ 1], first, sh, acc);
+    phase_bi_p<K, TI, kPlain>(I, row_lo, row_hi, (TI*)I.r, pb[par], pb[par ^ 1], vb[par], vb[par ^ 1], first, sh,
+                              acc);
   } else if constexpr (PH == PH_BI_S) {
     TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
     phase_bi_s<K, TI, kPlain>(I, row_lo, row_hi, (TI*)I.r, vb[par ^ 1], sh, acc);
   } else if constexpr (PH == PH_BI_X) {
     TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
     phase_bi_x<K, TI, kPlain>(I, row_lo, row_hi, (TI*)I.r, pb[par ^ 1], sh, acc);
-  } else {
-    phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step);
   }
   if constexpr (NS > 0) {
     block_partials<K, NS, kPlain>(acc, sh, sh.red, 0);
@@ -129,12 +132,117 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   }
 }
 
-// Sum of the instance's block partials, in block order: thread (q, c) sums
+// V adjacent columns per lane (stream_vec.h); g: the staged pattern of the
+// gathering Krylov phases.
+template <int K, int V, typename TI, int PH>
+__device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int row_lo, int row_hi,
+                           const Stage<TI>& g) {
+  constexpr int NS = nsums<PH>();
+  const bool first = a.first != 0;
+  const int par = a.par;
+  vec<double, V> acc[NS];
+  for (int q = 0; q < NS; q++) acc[q] = splat<V>(0.0);
+  if constexpr (PH == PH_RHS) {
+    vphase_rhs<K, V, TI>(I, row_lo, row_hi, a.step, acc);
+  } else if constexpr (PH == PH_RES) {
+    vphase_res<K, V, TI>(I, row_lo, row_hi, sh, acc);
+  } else if constexpr (PH == PH_PCG_INIT) {
+    vphase_pcg_init<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, (TI*)I.q, acc);
+  } else if constexpr (PH == PH_PCG_ITER) {
+    TI* rb[2] = {(TI*)I.r, (TI*)I.z};
+    TI* wb[2] = {(TI*)I.q, (TI*)I.rh};
+    TI* sb[2] = {(TI*)I.s, (TI*)I.t};
+    vphase_pcg_iter<K, V, TI>(I, row_lo, row_hi, g, rb[par], wb[par], sb[par], rb[par ^ 1], wb[par ^ 1],
+                              sb[par ^ 1], first, sh, acc);
+  } else if constexpr (PH == PH_CG_A) {
+    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
+    vphase_cg_a<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, pb[par], pb[par ^ 1], first, sh, acc);
+  } else if constexpr (PH == PH_CG_B) {
+    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
+    vphase_cg_b<K, V, TI>(I, row_lo, row_hi, (TI*)I.r, pb[par ^ 1], sh, acc);
+  } else if constexpr (PH == PH_BI_P) {
+    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
+    TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
+    vphase_bi_p<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, pb[par], pb[par ^ 1], vb[par], vb[par ^ 1], first, sh,
+                          acc);
+  } else if constexpr (PH == PH_BI_S) {
+    TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
+    vphase_bi_s<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, vb[par ^ 1], sh, acc);
+  } else {
+    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
+    vphase_bi_x<K, V, TI>(I, row_lo, row_hi, (TI*)I.r, pb[par ^ 1], sh, acc);
+  }
+  vblock_partials<K, V, NS>(acc, sh);
+  __syncthreads();
+  if (threadIdx.x < NS * K) a.partials[(size_t)b * 3 * K + threadIdx.x] = sh.red[threadIdx.x];
+}
+
+template <int K, typename TI, int PH>
+__global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
+  __shared__ Shared sh;
+  const int c = xcd_chunk(blockIdx.x, gridDim.x);
+  const int b = a.act_blk ? a.act_blk[c] : c;
+  const int i = a.blk_inst[b];
+  const Inst& I = a.insts[i];
+  const SState& S = a.ss[i];
+  // skip rules (uniform per block)
+  if constexpr (PH == PH_RES) {
+    if (a.pass ? !S.redo : S.solve_done) return;
+  } else if constexpr (PH != PH_RHS && PH != PH_UPD) {
+    if (S.sweep_done) return;
+  }
+  if (threadIdx.x < K) {
+    const int j = threadIdx.x;
+    sh.gamma[j] = S.gamma[j];
+    sh.alpha[j] = S.alpha[j];
+    sh.beta[j] = S.beta[j];
+    sh.omega[j] = S.omega[j];
+    sh.bb[j] = S.bb[j];
+    sh.reset[j] = (PH == PH_RES && a.pass) ? S.reset[j] : 0;
+  }
+  const int row_lo = a.blk_row[b];
+  const int row_hi = a.blk_end[b];
+  constexpr int V = vec_cols<K, TI>();
+  if constexpr (PH == PH_UPD) {
+    __syncthreads();
+    phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step);
+  } else if constexpr (V > 1 && staged_phase<PH>()) {
+    // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
+    extern __shared__ int dyn[];
+    const int GA* rp = gp(I.rp);
+    const int s_lo = rp[row_lo], s_hi = rp[row_hi];
+    const int nrow = row_hi - row_lo, nnz = s_hi - s_lo;  // within the stage (plan)
+    int LA* lrp = (int LA*)dyn;
+    int LA* lci = lrp + ((a.stage_rows + 4) & ~3);
+    TI LA* lw = (TI LA*)(lci + ((a.stage_nnz + 3) & ~3));
+    const int GA* ci = gp(I.ci);
+    const TI GA* w = gp((const TI*)(PH == PH_CG_A ? I.Si : I.SDi));
+    for (int t = threadIdx.x; t <= nrow; t += kBlock) lrp[t] = rp[row_lo + t] - s_lo;
+    for (int t = threadIdx.x; t < nnz; t += kBlock) {
+      lci[t] = ci[s_lo + t];
+      lw[t] = w[s_lo + t];
+    }
+    __syncthreads();
+    run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
+  } else if constexpr (V > 1) {
+    __syncthreads();
+    run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
+  } else {
+    __syncthreads();
+    run_scalar<K, TI, PH>(I, a, sh, b, row_lo, row_hi);
+  }
+}
+
+// Threads of a reduction workgroup: the largest instances have thousands of
+// row blocks, and the sum is bound by how many partials are in flight.
+constexpr int kRedBlock = 1024;
+
+// Sum of the instance's block partials in a fixed order: thread (q, c) sums
 // blocks c, c + chunks, ... of value q; the chunk sums are combined in order.
 template <int K, int NS>
 __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* red, double* scratch) {
   constexpr int nv = NS * K;
-  constexpr int chunks = kBlock / nv;
+  constexpr int chunks = kRedBlock / nv;
   const int q = threadIdx.x % nv, c = threadIdx.x / nv;
   const int b0 = inst_blk[i], b1 = inst_blk[i + 1];
   double s = 0.0;
@@ -164,11 +272,11 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
 // One block per instance: reduce the phase's partials and advance the scalar
 // recurrences exactly as the persistent drivers do after their barriers.
 template <int K, int PH>
-__global__ void __launch_bounds__(kBlock) k_sreduce(SArgs a, const int* inst_blk, int ninst, int fp32) {
-  __shared__ double scratch[kBlock];
+__global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_blk, int ninst, int fp32) {
+  __shared__ double scratch[kRedBlock];
   __shared__ double red[3 * kMaxK];
-  const int i = blockIdx.x;
-  if (i >= ninst) return;
+  if ((int)blockIdx.x >= ninst) return;
+  const int i = a.act_inst ? a.act_inst[blockIdx.x] : blockIdx.x;
   const Inst& I = a.insts[i];
   SState& S = a.ss[i];
   if constexpr (PH == PH_RES) {
@@ -347,16 +455,60 @@ __global__ void k_sinit(SState* ss, const Inst* insts, int ninst) {
   *insts[i].status = 0;
 }
 
-// *out = 1 when every instance has `field` set (0: sweep_done, 1: solve_done).
-__global__ void k_all(const SState* ss, int ninst, int field, int* out) {
-  __shared__ int any;
-  if (threadIdx.x == 0) any = 0;
+// Instances that take part in the next launches: pred 0 the sweep is running,
+// 1 the solve is running, 2 the residual pass must be redone.  act_base[i] is
+// the instance's first position in the active block list (-1: inactive),
+// act_inst lists the active instances in order, cnt = {instances, blocks}.
+// One workgroup; instances in chunks of kBlock with a running offset.
+__global__ void __launch_bounds__(kBlock) k_compact(const SState* ss, const int* inst_blk, int ninst, int pred,
+                                                    int* act_inst, int* act_base, int* cnt) {
+  __shared__ int sb[kBlock], si[kBlock];
+  __shared__ int tot_b, tot_i;
+  const int t = threadIdx.x;
+  if (t == 0) tot_b = tot_i = 0;
   __syncthreads();
-  int mine = 0;
-  for (int i = threadIdx.x; i < ninst; i += blockDim.x) mine |= !(field ? ss[i].solve_done : ss[i].sweep_done);
-  if (mine) atomicOr(&any, 1);
-  __syncthreads();
-  if (threadIdx.x == 0) *out = !any;
+  for (int i0 = 0; i0 < ninst; i0 += kBlock) {
+    const int i = i0 + t;
+    int on = 0, nb = 0;
+    if (i < ninst) {
+      const SState& S = ss[i];
+      on = pred == 0 ? !S.sweep_done : pred == 1 ? !S.solve_done : S.redo != 0;
+      nb = on ? inst_blk[i + 1] - inst_blk[i] : 0;
+    }
+    sb[t] = nb;
+    si[t] = on;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {  // inclusive scans
+      const int vb = t >= d ? sb[t - d] : 0, vi = t >= d ? si[t - d] : 0;
+      __syncthreads();
+      sb[t] += vb;
+      si[t] += vi;
+      __syncthreads();
+    }
+    if (i < ninst) {
+      act_base[i] = on ? tot_b + sb[t] - nb : -1;
+      if (on) act_inst[tot_i + si[t] - 1] = i;
+    }
+    __syncthreads();
+    if (t == kBlock - 1) {
+      tot_b += sb[t];
+      tot_i += si[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    cnt[0] = tot_i;
+    cnt[1] = tot_b;
+  }
+}
+
+// Active block list: every row block of an active instance, in row-block order.
+__global__ void k_expand(const int* blk_inst, const int* inst_blk, const int* act_base, int nblocks, int* act_blk) {
+  const int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= nblocks) return;
+  const int i = blk_inst[gb];
+  const int base = act_base[i];
+  if (base >= 0) act_blk[base + gb - inst_blk[i]] = gb;
 }
 
 template <int K, typename TI>
@@ -419,54 +571,90 @@ const void* pick_reduce(int k, int ph) {
 
 struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
+  int stage_rows = 0, stage_nnz = 0;  // largest row block: rows, nonzeros (up to kStageCap)
+  size_t lds = 0;                     // dynamic LDS of the staged phases
   SState* d_ss = nullptr;
   int* d_blk_inst = nullptr;
   int* d_blk_row = nullptr;
+  int* d_blk_end = nullptr;
   int* d_inst_blk = nullptr;
   double* d_partials = nullptr;
-  int* d_flag = nullptr;
-  int* h_flag = nullptr;  // pinned
+  int* d_act_inst = nullptr;  // [ninst]
+  int* d_act_base = nullptr;  // [ninst]
+  int* d_act_blk = nullptr;   // [nblocks]
+  int* d_cnt = nullptr;       // {active instances, active blocks}
+  int* h_cnt = nullptr;       // pinned copy
 };
 
 void stream_plan_free(StreamPlan* p) {
   if (!p) return;
-  void* ptrs[] = {p->d_ss, p->d_blk_inst, p->d_blk_row, p->d_inst_blk, p->d_partials, p->d_flag};
+  void* ptrs[] = {p->d_ss,       p->d_blk_inst, p->d_blk_row,  p->d_blk_end, p->d_inst_blk,
+                  p->d_partials, p->d_act_inst, p->d_act_base, p->d_act_blk,  p->d_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (p->h_flag) (void)hipHostFree(p->h_flag);
+  if (p->h_cnt) (void)hipHostFree(p->h_cnt);
   delete p;
 }
 
-// Row-block table over all instances; `rows` per block (a multiple of 256/k).
+// Row-block table over all instances: up to `rows` rows per block, cut
+// shorter where the block's pattern would exceed kStageCap nonzeros.
 int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
                       hipStream_t st, StreamPlan** out) {
   (void)d_insts;
+  if (rows < 1) {
+    set_error("stream engine: rows per block must be positive");
+    return FEM_ERR_INVALID;
+  }
   StreamPlan* p = new StreamPlan();
   p->ninst = (int)insts.size();
   p->k = k;
   p->fp32 = fp32;
   p->kr = kr;
   p->rows = rows;
-  std::vector<int> bi, br, ib(p->ninst + 1, 0);
+  std::vector<int> bi, br, be, ib(p->ninst + 1, 0);
   for (int i = 0; i < p->ninst; i++) {
     ib[i] = (int)bi.size();
-    const int n = insts[i].ctx->n;
-    for (int r = 0; r < n; r += rows) {
+    const fem_ctx* c = insts[i].ctx;
+    const int n = c->n;
+    const std::vector<int>& rp = c->h_rp;
+    if ((int)rp.size() != n + 1) {
+      delete p;
+      set_error("stream engine: context pattern not assembled");
+      return FEM_ERR_STATE;
+    }
+    for (int r = 0; r < n;) {
+      int e = std::min(n, r + rows);
+      while (e > r + 1 && rp[e] - rp[r] > kStageCap) e--;
       bi.push_back(i);
       br.push_back(r);
+      be.push_back(e);
+      p->stage_rows = std::max(p->stage_rows, e - r);
+      p->stage_nnz = std::max(p->stage_nnz, rp[e] - rp[r]);
+      r = e;
     }
   }
   ib[p->ninst] = (int)bi.size();
   p->nblocks = (int)bi.size();
+  const size_t tb = fp32 ? 4 : 8;
+  p->lds = 4 * (size_t)((p->stage_rows + 4) & ~3) + 4 * (size_t)((p->stage_nnz + 3) & ~3) + tb * p->stage_nnz;
+  if (k >= 4 && p->lds > kMaxLds) {
+    delete p;
+    set_error("stream engine: a matrix row has too many nonzeros for the LDS stage; use the persistent engine");
+    return FEM_ERR_INVALID;
+  }
   int rc = 0;
   p->d_ss = dalloc<SState>(p->ninst, &rc);
   p->d_blk_inst = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_blk_row = dalloc<int>(std::max(1, p->nblocks), &rc);
+  p->d_blk_end = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_inst_blk = dalloc<int>(p->ninst + 1, &rc);
   p->d_partials = dalloc<double>((size_t)std::max(1, p->nblocks) * 3 * k, &rc);
-  p->d_flag = dalloc<int>(1, &rc);
+  p->d_act_inst = dalloc<int>(p->ninst, &rc);
+  p->d_act_base = dalloc<int>(p->ninst, &rc);
+  p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);
+  p->d_cnt = dalloc<int>(2, &rc);
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&p->h_flag, sizeof(int), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&p->h_cnt, 2 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
   }
   if (rc) {
@@ -476,6 +664,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   if (p->nblocks > 0) {
     FEM_HIP(hipMemcpyAsync(p->d_blk_inst, bi.data(), sizeof(int) * bi.size(), hipMemcpyHostToDevice, st));
     FEM_HIP(hipMemcpyAsync(p->d_blk_row, br.data(), sizeof(int) * br.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(hipMemcpyAsync(p->d_blk_end, be.data(), sizeof(int) * be.size(), hipMemcpyHostToDevice, st));
   }
   FEM_HIP(hipMemcpyAsync(p->d_inst_blk, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice, st));
   FEM_HIP(hipStreamSynchronize(st));
@@ -485,32 +674,54 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
 
 namespace {
 
-int launch_phase(const StreamPlan* p, const SArgs& a, int ph, hipStream_t st) {
+// The blocks and instances one launch covers: every row block and instance
+// (all = true), or the lists of the last compaction.
+struct Cover {
+  bool all;
+  int nact, nab;
+};
+
+int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, hipStream_t st) {
   const void* fn = pick_phase(p->k, p->fp32, ph);
   if (!fn) {
     set_error("stream engine: unsupported column count");
     return FEM_ERR_INVALID;
   }
-  if (p->nblocks == 0) return 0;
+  const int nab = cv.all ? p->nblocks : cv.nab;
+  const int nact = cv.all ? p->ninst : cv.nact;
+  if (nab == 0) return 0;
   SArgs aa = a;
+  aa.act_blk = cv.all ? nullptr : p->d_act_blk;
+  aa.act_inst = cv.all ? nullptr : p->d_act_inst;
   void* args[] = {(void*)&aa};
-  FEM_HIP(hipLaunchKernel(fn, dim3(p->nblocks), dim3(kBlock), args, 0, st));
+  const bool staged = p->k >= 4 && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A || ph == PH_BI_P ||
+                                    ph == PH_BI_S);
+  FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, staged ? p->lds : 0, st));
   if (ph == PH_UPD) return 0;
   const void* rf = pick_reduce(p->k, ph);
   const int* ib = p->d_inst_blk;
-  int ninst = p->ninst, fp32 = p->fp32;
-  void* rargs[] = {(void*)&aa, (void*)&ib, (void*)&ninst, (void*)&fp32};
-  FEM_HIP(hipLaunchKernel(rf, dim3(p->ninst), dim3(kBlock), rargs, 0, st));
+  int fp32 = p->fp32;
+  void* rargs[] = {(void*)&aa, (void*)&ib, (void*)&nact, (void*)&fp32};
+  FEM_HIP(hipLaunchKernel(rf, dim3(nact), dim3(kRedBlock), rargs, 0, st));
   return 0;
 }
 
-// Host read of "every instance has finished its sweep (0) / solve (1)".
-int all_done(const StreamPlan* p, int field, hipStream_t st, bool* done) {
-  hipLaunchKernelGGL(k_all, dim3(1), dim3(kBlock), 0, st, p->d_ss, p->ninst, field, p->d_flag);
+// Compact the instances and row blocks that satisfy `pred` (k_compact) and
+// read the counts back: nab == 0 means none is left.
+int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv) {
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(kBlock), 0, st, p->d_ss, p->d_inst_blk, p->ninst, pred, p->d_act_inst,
+                     p->d_act_base, p->d_cnt);
   FEM_HIP(hipGetLastError());
-  FEM_HIP(hipMemcpyAsync(p->h_flag, p->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  FEM_HIP(hipMemcpyAsync(p->h_cnt, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (p->nblocks > 0) {  // queued before the host waits, so it overlaps the read-back
+    hipLaunchKernelGGL(k_expand, dim3(div_up(p->nblocks, kBlock)), dim3(kBlock), 0, st, p->d_blk_inst, p->d_inst_blk,
+                       p->d_act_base, p->nblocks, p->d_act_blk);
+    FEM_HIP(hipGetLastError());
+  }
   FEM_HIP(hipStreamSynchronize(st));
-  *done = *p->h_flag != 0;
+  cv->all = false;
+  cv->nact = p->h_cnt[0];
+  cv->nab = p->h_cnt[1];
   return 0;
 }
 
@@ -525,65 +736,70 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   a.ss = p->d_ss;
   a.blk_inst = p->d_blk_inst;
   a.blk_row = p->d_blk_row;
-  a.rows = p->rows;
+  a.blk_end = p->d_blk_end;
+  a.stage_rows = p->stage_rows;
+  a.stage_nnz = p->stage_nnz;
   a.partials = p->d_partials;
   hipLaunchKernelGGL(k_sinit, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts, p->ninst);
   FEM_HIP(hipGetLastError());
   int rc = 0;
-#define STREAM_PH(ph)                                   \
-  do {                                                  \
-    if ((rc = launch_phase(p, a, (ph), st))) return rc; \
+  const Cover full{true, 0, 0};
+  Cover cv{};
+#define STREAM_PH(ph, cover)                                     \
+  do {                                                           \
+    if ((rc = launch_phase(p, a, (ph), (cover), st))) return rc; \
   } while (0)
   for (int step = 0; step < nsteps; step++) {
     a.step = step;
     a.par = 0;
     a.first = 1;
     a.pass = 0;
-    STREAM_PH(PH_RHS);
+    STREAM_PH(PH_RHS, full);
     for (int sw = 0; sw < max_sweeps; sw++) {
       hipLaunchKernelGGL(k_sweep_begin, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts,
                          p->ninst, p->k, p->fp32);
       FEM_HIP(hipGetLastError());
-      bool done = false;
-      if ((rc = all_done(p, 1, st, &done))) return rc;
-      if (done) break;
+      if ((rc = compact(p, 0, st, &cv))) return rc;  // instances that sweep
+      if (cv.nab == 0) break;
       int par = 0;
       if (p->kr == 0) {  // pipelined CG
         a.par = 0;
         a.first = 1;
-        STREAM_PH(PH_PCG_INIT);
+        STREAM_PH(PH_PCG_INIT, cv);
         for (int it = 0; it < max_inner; it++) {
           a.par = par;
           a.first = it == 0;
-          STREAM_PH(PH_PCG_ITER);
+          STREAM_PH(PH_PCG_ITER, cv);
           par ^= 1;
-          if ((rc = all_done(p, 0, st, &done))) return rc;
-          if (done) break;
+          if ((rc = compact(p, 0, st, &cv))) return rc;
+          if (cv.nab == 0) break;
         }
       } else {  // classic CG (2) or BiCGSTAB (1): p / v buffers swap every iteration
         for (int it = 0; it < max_inner; it++) {
           a.par = par;
           a.first = it == 0;
           if (p->kr == 2) {
-            STREAM_PH(PH_CG_A);
-            STREAM_PH(PH_CG_B);
+            STREAM_PH(PH_CG_A, cv);
+            STREAM_PH(PH_CG_B, cv);
           } else {
-            STREAM_PH(PH_BI_P);
-            STREAM_PH(PH_BI_S);
-            STREAM_PH(PH_BI_X);
+            STREAM_PH(PH_BI_P, cv);
+            STREAM_PH(PH_BI_S, cv);
+            STREAM_PH(PH_BI_X, cv);
           }
           par ^= 1;
-          if ((rc = all_done(p, 0, st, &done))) return rc;
-          if (done) break;
+          if ((rc = compact(p, 0, st, &cv))) return rc;
+          if (cv.nab == 0) break;
         }
       }
       a.pass = 0;
-      STREAM_PH(PH_RES);
+      if ((rc = compact(p, 1, st, &cv))) return rc;  // every instance of this sweep
+      STREAM_PH(PH_RES, cv);
+      if ((rc = compact(p, 2, st, &cv))) return rc;  // columns that produced non-finite values
       a.pass = 1;
-      STREAM_PH(PH_RES);
+      STREAM_PH(PH_RES, cv);
       a.pass = 0;
     }
-    STREAM_PH(PH_UPD);
+    STREAM_PH(PH_UPD, full);
     hipLaunchKernelGGL(k_step_end, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts, p->ninst,
                        p->k);
     FEM_HIP(hipGetLastError());

@@ -119,7 +119,8 @@ typedef struct fem_solver_opts {
   int32_t max_iters;     /* Krylov iterations per solve; 0 = 2N (Eigen's default) */
   int32_t max_inner;     /* FP32 iterations per refinement sweep; 0 = 200 */
   int32_t max_sweeps;    /* refinement sweeps per solve; 0 = 30 */
-  int32_t team_size;     /* workgroups per instance; 0 = autotuned */
+  int32_t team_size;     /* persistent engine: workgroups per instance; stream engine: rows per
+                            workgroup; 0 = autotuned / by column count */
   double tol;            /* relative residual |b - Ax| <= tol |b| in FP64; 0 = 1e-10 */
   double inner_tol;      /* FP32 sweep reduction; 0 = 1e-4 */
   int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or

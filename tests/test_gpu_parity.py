@@ -417,6 +417,61 @@ def test_stream_engine_matches_oracle(gpu, case):
     assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
 
 
+STREAM_MULTI = [
+    ("heat", _lib.FEM_KRYLOV_CG, 4, _lib.FEM_PREC_FP32_IR),
+    ("heat", _lib.FEM_KRYLOV_CG_CLASSIC, 16, _lib.FEM_PREC_FP64),
+    ("heat", _lib.FEM_KRYLOV_CG, 16, _lib.FEM_PREC_FP64),
+    ("wave", _lib.FEM_KRYLOV_CG, 4, _lib.FEM_PREC_FP64),
+    ("newmark", _lib.FEM_KRYLOV_CG_CLASSIC, 16, _lib.FEM_PREC_FP32_IR),
+    ("advdiff", _lib.FEM_KRYLOV_BICGSTAB, 4, _lib.FEM_PREC_FP64),
+]
+
+
+@pytest.mark.parametrize("eq,krylov,k,prec", STREAM_MULTI)
+def test_stream_engine_multi_rhs(gpu, eq, krylov, k, prec):
+    """The streaming engine's multi-column phase bodies (each lane owns 16
+    bytes of adjacent columns) for every Krylov method, both inner
+    precisions and the heat, wave, Newmark and advection-diffusion
+    right-hand sides, column by column against the oracle on two uneven
+    meshes."""
+    print("case stream multi", eq, krylov, k, prec, flush=True)
+    rng = np.random.default_rng(41 + k)
+    vel = (0.5, -0.3, 0.0)
+    dt, T = 2e-3, 4
+    ctxs, systems, u0s, a0s = [], [], [], []
+    for side, seed in ((15, 1), (23, 2)):
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+        s = oracle.System(m, bc=0, velocity=vel, normalize=False)
+        systems.append(s)
+        u0 = rng.normal(size=(s.n, k))
+        u0s.append(u0)
+        if eq == "newmark":
+            a0s.append(np.stack([s.newmark_init(1.0, u0[:, j]) for j in range(k)], axis=1))
+    b = Batch(ctxs, SolverOptions(nrhs=k, engine=_lib.FEM_ENGINE_STREAM, krylov=krylov, precision=prec))
+    params = {"heat": (HeatParameters(dt, 0.05), oracle.HEAT, [dt, 0.05]),
+              "wave": (WaveParameters(dt, 0.9), oracle.WAVE, [dt, 0.9]),
+              "newmark": (NewmarkParameters(dt, 1.0, 0.25, 0.5), oracle.NEWMARK, [dt, 1.0, 0.25, 0.5]),
+              "advdiff": (AdvectionDiffusionParameters(dt, 2e-2), oracle.ADVDIFF, [dt, 2e-2])}[eq]
+    for i in range(len(ctxs)):
+        b.set_equation(i, params[0])
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, u0s[i])
+        if eq == "newmark":
+            b.set_state_unknowns(i, _lib.FEM_FIELD_A, a0s[i])
+    assert b.advance(T) == _lib.FEM_OK
+    okr = oracle.BICGSTAB if krylov == _lib.FEM_KRYLOV_BICGSTAB else oracle.CG
+    for i, s in enumerate(systems):
+        got = b.get_state_unknowns(i).reshape(s.n, k)
+        gotv = b.get_state_unknowns(i, _lib.FEM_FIELD_V).reshape(s.n, k)
+        for j in range(k):
+            a0 = a0s[i][:, j].copy() if eq == "newmark" else None
+            _, u, v, _ = s.run(params[1], params[2], T, u0s[i][:, j].copy(), None, a0, krylov=okr, tol=1e-14)
+            assert rel(got[:, j], u) < RTOL, (i, j)
+            if eq in ("wave", "newmark"):
+                assert rel(gotv[:, j], v) < RTOL, (i, j)
+    b.close()
+
+
 def test_stream_engine_adjoint_matches_persistent(gpu):
     from femhip import Adjoint
     m = meshes.jittered_square(24, seed=12)
