@@ -263,7 +263,10 @@ def run_heat_c1(args):
     out = contract_line(METRIC, n * steps / wall, "DOF·steps/s", steps, args.warmup, 1e3 * wall / steps, "f64",
                         {"workload": f"heat, structured 64x64 unit square (N={n}, nnz={nnz}), dt=1e-3, "
                                      f"conductivity 1, Dirichlet, Gaussian u0, FP64 CG tol 1e-10",
-                         "global_batch": 1, "parallelism": "single GPU, persistent cooperative kernel"},
+                         "global_batch": 1,
+                         "parallelism": "single GPU, " + ("stream engine (autotuned)"
+                                                          if sol.options.engine == _lib.FEM_ENGINE_STREAM
+                                                          else "persistent cooperative kernel")},
                         status=int(status),
                         solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
                                 "kernel_ms": st.ms},
@@ -309,9 +312,13 @@ def run_wave_c3(args):
         sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
         sol.init_newmark_acceleration()
         return sol
-    sol = fresh(tol=1e-10)
+    if args.engine == "stream":
+        sol = fresh(tol=1e-10, engine=_lib.FEM_ENGINE_STREAM)
+        tuned = None
+    else:
+        sol = fresh(tol=1e-10)
+        tuned = sol.autotune(5) if not args.no_autotune else None
     a0 = sol.get_state_unknowns(_lib.FEM_FIELD_A)
-    tuned = sol.autotune(5) if not args.no_autotune else None
     sol.advance(args.warmup)
     sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
     sol.set_state_unknowns(_lib.FEM_FIELD_V, np.zeros(n))
@@ -330,7 +337,10 @@ def run_wave_c3(args):
                         "f32+f64 (FP32 CG, FP64 refinement residual and dots)",
                         {"workload": f"Newmark wave, jittered 1024x1024 lattice (N={n}, nnz={nnz}), c=1, "
                                      f"beta=1/4 gamma=1/2, dt=2e-4, 8 Gaussian bumps, tol 1e-10",
-                         "global_batch": 1, "parallelism": "single GPU, persistent cooperative kernel"},
+                         "global_batch": 1,
+                         "parallelism": "single GPU, " + ("stream engine (autotuned)"
+                                                          if sol.options.engine == _lib.FEM_ENGINE_STREAM
+                                                          else "persistent cooperative kernel")},
                         status=int(status),
                         solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
                                 "kernel_ms": st.ms, "iters_per_step": st.iterations / steps},
@@ -578,7 +588,7 @@ def main():
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
-                    help="batched: 'stream' runs the streaming engine without autotuning")
+                    help="batched, wave: 'stream' runs the streaming engine without autotuning")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-spmv", action="store_true")

@@ -121,7 +121,7 @@ int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
   h.d_Si = (void*)dalloc<char>((size_t)nnz * kv * tb, &rc);
   h.d_dinv64 = dalloc<double>((size_t)n * kv, &rc);
   h.d_dinvi = (void*)dalloc<char>((size_t)n * kv * tb, &rc);
-  if (kv == 1 && k >= 4) h.d_SDi = (void*)dalloc<char>((size_t)nnz * tb, &rc);
+  if (kv == 1 && stream_vec_layout(k)) h.d_SDi = (void*)dalloc<char>((size_t)nnz * tb, &rc);
   h.d_u = dalloc<double>((size_t)n * k, &rc);
   h.d_v = dalloc<double>((size_t)n * k, &rc);
   h.d_a = dalloc<double>((size_t)n * k, &rc);
@@ -866,8 +866,8 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
       set_error("batch instances must share the number of columns and the Krylov method");
       return FEM_ERR_INVALID;
     }
-    if (k >= 4 && (h.kv != 1 || !h.d_SDi)) {  // the 16-byte column layout shares one matrix across columns
-      set_error("stream engine: instances of four or more columns need one matrix for all columns");
+    if (stream_vec_layout(k) && (h.kv != 1 || !h.d_SDi)) {  // the column layout shares one matrix across columns
+      set_error("stream engine: instances of one or of four or more columns need one matrix for all columns");
       return FEM_ERR_INVALID;
     }
   }

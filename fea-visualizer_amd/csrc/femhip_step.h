@@ -32,6 +32,9 @@ void stream_plan_free(StreamPlan* p);
 int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
                       hipStream_t st, StreamPlan** out);
 int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps, hipStream_t st);
+// Column counts whose stream phases use the multi-column layout with the
+// staged pattern (one value column, Inst::SDi present).
+bool stream_vec_layout(int k);
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
 
 // Per-instance host bookkeeping.

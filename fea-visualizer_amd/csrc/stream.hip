@@ -61,6 +61,7 @@ struct SArgs {
   double* partials;     // [nblocks][3 K]
   const int* act_blk;   // launched block -> row block (null: every row block, in order)
   const int* act_inst;  // reduction block -> instance (null: every instance)
+  const int* act_cnt;   // {instances, blocks} of the lists (device); the grid may be larger
   const int* blk_end;   // [nblocks] end row
   int stage_rows;       // LDS stage capacity: rows and nonzeros per row block
   int stage_nnz;
@@ -180,7 +181,12 @@ __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int
 template <int K, typename TI, int PH>
 __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   __shared__ Shared sh;
-  const int c = xcd_chunk(blockIdx.x, gridDim.x);
+  int nb = gridDim.x;
+  if (a.act_cnt) {  // the grid is the host's last count, a superset of the current list
+    nb = a.act_cnt[1];
+    if ((int)blockIdx.x >= nb) return;
+  }
+  const int c = xcd_chunk(blockIdx.x, nb);
   const int b = a.act_blk ? a.act_blk[c] : c;
   const int i = a.blk_inst[b];
   const Inst& I = a.insts[i];
@@ -206,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   if constexpr (PH == PH_UPD) {
     __syncthreads();
     phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step);
-  } else if constexpr (V > 1 && staged_phase<PH>()) {
+  } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
     // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
     extern __shared__ int dyn[];
     const int GA* rp = gp(I.rp);
@@ -224,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
     }
     __syncthreads();
     run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
-  } else if constexpr (V > 1) {
+  } else if constexpr (vec_layout<K>()) {
     __syncthreads();
     run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
   } else {
@@ -275,7 +281,7 @@ template <int K, int PH>
 __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_blk, int ninst, int fp32) {
   __shared__ double scratch[kRedBlock];
   __shared__ double red[3 * kMaxK];
-  if ((int)blockIdx.x >= ninst) return;
+  if ((int)blockIdx.x >= ninst || (a.act_cnt && (int)blockIdx.x >= a.act_cnt[0])) return;
   const int i = a.act_inst ? a.act_inst[blockIdx.x] : blockIdx.x;
   const Inst& I = a.insts[i];
   SState& S = a.ss[i];
@@ -569,6 +575,8 @@ const void* pick_reduce(int k, int ph) {
 // Host side
 // ---------------------------------------------------------------------------
 
+bool stream_vec_layout(int k) { return k == 1 || k >= 4; }
+
 struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
   int stage_rows = 0, stage_nnz = 0;  // largest row block: rows, nonzeros (up to kStageCap)
@@ -637,7 +645,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->nblocks = (int)bi.size();
   const size_t tb = fp32 ? 4 : 8;
   p->lds = 4 * (size_t)((p->stage_rows + 4) & ~3) + 4 * (size_t)((p->stage_nnz + 3) & ~3) + tb * p->stage_nnz;
-  if (k >= 4 && p->lds > kMaxLds) {
+  if (stream_vec_layout(k) && p->lds > kMaxLds) {
     delete p;
     set_error("stream engine: a matrix row has too many nonzeros for the LDS stage; use the persistent engine");
     return FEM_ERR_INVALID;
@@ -675,11 +683,18 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
 namespace {
 
 // The blocks and instances one launch covers: every row block and instance
-// (all = true), or the lists of the last compaction.
+// (all = true), or the lists of the last compaction, launched over nab blocks
+// and nact reduction workgroups (the last counts the host read: a superset of
+// the lists, whose own counts the kernels read on the device).
 struct Cover {
   bool all;
   int nact, nab;
 };
+
+// Host checks of "anything left" in the Krylov loop: every kCheckEvery-th
+// iteration.  Instances only ever leave the lists, so between checks the
+// launches cover a superset and finished blocks exit at once.
+constexpr int kCheckEvery = 4;
 
 int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, hipStream_t st) {
   const void* fn = pick_phase(p->k, p->fp32, ph);
@@ -693,9 +708,10 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   SArgs aa = a;
   aa.act_blk = cv.all ? nullptr : p->d_act_blk;
   aa.act_inst = cv.all ? nullptr : p->d_act_inst;
+  aa.act_cnt = cv.all ? nullptr : p->d_cnt;
   void* args[] = {(void*)&aa};
-  const bool staged = p->k >= 4 && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A || ph == PH_BI_P ||
-                                    ph == PH_BI_S);
+  const bool staged = stream_vec_layout(p->k) && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A ||
+                                                  ph == PH_BI_P || ph == PH_BI_S);
   FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, staged ? p->lds : 0, st));
   if (ph == PH_UPD) return 0;
   const void* rf = pick_reduce(p->k, ph);
@@ -706,22 +722,25 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   return 0;
 }
 
-// Compact the instances and row blocks that satisfy `pred` (k_compact) and
-// read the counts back: nab == 0 means none is left.
-int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv) {
+// Compact the instances and row blocks that satisfy `pred` into the device
+// lists (k_compact, k_expand).  With `read`, wait for the counts and make them
+// the launch sizes: nab == 0 means none is left.
+int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, bool read) {
   hipLaunchKernelGGL(k_compact, dim3(1), dim3(kBlock), 0, st, p->d_ss, p->d_inst_blk, p->ninst, pred, p->d_act_inst,
                      p->d_act_base, p->d_cnt);
   FEM_HIP(hipGetLastError());
-  FEM_HIP(hipMemcpyAsync(p->h_cnt, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (read) FEM_HIP(hipMemcpyAsync(p->h_cnt, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   if (p->nblocks > 0) {  // queued before the host waits, so it overlaps the read-back
     hipLaunchKernelGGL(k_expand, dim3(div_up(p->nblocks, kBlock)), dim3(kBlock), 0, st, p->d_blk_inst, p->d_inst_blk,
                        p->d_act_base, p->nblocks, p->d_act_blk);
     FEM_HIP(hipGetLastError());
   }
-  FEM_HIP(hipStreamSynchronize(st));
   cv->all = false;
-  cv->nact = p->h_cnt[0];
-  cv->nab = p->h_cnt[1];
+  if (read) {
+    FEM_HIP(hipStreamSynchronize(st));
+    cv->nact = p->h_cnt[0];
+    cv->nab = p->h_cnt[1];
+  }
   return 0;
 }
 
@@ -745,6 +764,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   int rc = 0;
   const Cover full{true, 0, 0};
   Cover cv{};
+  const bool multi = p->ninst > 1;  // one instance: its list is all of its blocks or none
 #define STREAM_PH(ph, cover)                                     \
   do {                                                           \
     if ((rc = launch_phase(p, a, (ph), (cover), st))) return rc; \
@@ -759,23 +779,40 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       hipLaunchKernelGGL(k_sweep_begin, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts,
                          p->ninst, p->k, p->fp32);
       FEM_HIP(hipGetLastError());
-      if ((rc = compact(p, 0, st, &cv))) return rc;  // instances that sweep
+      if ((rc = compact(p, 0, st, &cv, true))) return rc;  // instances that sweep
       if (cv.nab == 0) break;
+      const Cover sweep = cv;  // covers every instance of this sweep
+      // one iteration done: refresh the lists (several instances) and, every
+      // kCheckEvery-th time, read whether any instance still iterates
+      int since = 0;
+      auto next = [&](bool* stop) -> int {
+        const bool check = ++since >= kCheckEvery;
+        *stop = false;
+        if (check || multi) {
+          const int r = compact(p, 0, st, &cv, check);
+          if (r) return r;
+        }
+        if (check) {
+          since = 0;
+          *stop = cv.nab == 0;
+        }
+        return 0;
+      };
+      bool stop = false;
       int par = 0;
       if (p->kr == 0) {  // pipelined CG
         a.par = 0;
         a.first = 1;
         STREAM_PH(PH_PCG_INIT, cv);
-        for (int it = 0; it < max_inner; it++) {
+        for (int it = 0; it < max_inner && !stop; it++) {
           a.par = par;
           a.first = it == 0;
           STREAM_PH(PH_PCG_ITER, cv);
           par ^= 1;
-          if ((rc = compact(p, 0, st, &cv))) return rc;
-          if (cv.nab == 0) break;
+          if ((rc = next(&stop))) return rc;
         }
       } else {  // classic CG (2) or BiCGSTAB (1): p / v buffers swap every iteration
-        for (int it = 0; it < max_inner; it++) {
+        for (int it = 0; it < max_inner && !stop; it++) {
           a.par = par;
           a.first = it == 0;
           if (p->kr == 2) {
@@ -787,14 +824,16 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
             STREAM_PH(PH_BI_X, cv);
           }
           par ^= 1;
-          if ((rc = compact(p, 0, st, &cv))) return rc;
-          if (cv.nab == 0) break;
+          if ((rc = next(&stop))) return rc;
         }
       }
+      // FP64 residual over the sweep's instances (device lists, no host wait),
+      // then the redo pass for columns that produced non-finite values
+      cv = sweep;
       a.pass = 0;
-      if ((rc = compact(p, 1, st, &cv))) return rc;  // every instance of this sweep
+      if ((rc = compact(p, 1, st, &cv, false))) return rc;
       STREAM_PH(PH_RES, cv);
-      if ((rc = compact(p, 2, st, &cv))) return rc;  // columns that produced non-finite values
+      if ((rc = compact(p, 2, st, &cv, false))) return rc;
       a.pass = 1;
       STREAM_PH(PH_RES, cv);
       a.pass = 0;

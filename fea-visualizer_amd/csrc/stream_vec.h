@@ -1,15 +1,16 @@
-// stream_vec.h -- phase bodies of the streaming engine for instances of 4 or
-// more right-hand sides.  Each lane owns V adjacent columns of a row (16 bytes
-// of the inner precision), so a row's neighbour values arrive in K / V
-// 16-byte loads instead of K scalar ones, and the column index and matrix
-// entry of a nonzero are loaded by K / V lanes instead of K.
+// stream_vec.h -- phase bodies of the streaming engine for instances of one
+// or of four and more right-hand sides.  Each lane owns V adjacent columns of
+// a row (up to 16 bytes of the inner precision), so a row's neighbour values
+// arrive in K / V loads of up to 16 bytes instead of K scalar ones, and the
+// column index and matrix entry of a nonzero are loaded by K / V lanes
+// instead of K.
 //
 // Per element the arithmetic is the persistent kernel's (phase_* in
 // step_kernel.h): the same expressions in the same precision and the same
 // ascending summation over each row.  Only the order in which the per-lane
 // partial dot products are combined differs.  The matrix and preconditioner
 // are shared by all columns (one value column), which is the case for every
-// instance with more than two right-hand sides.
+// instance with one or with more than two right-hand sides.
 #pragma once
 
 #include "step_kernel.h"
@@ -25,10 +26,18 @@ struct VecOf {
 template <typename T, int V>
 using vec = typename VecOf<T, V>::type;
 
-// Columns per lane: 16 bytes of the inner precision, at most K; one for K < 4.
+// Instances with one column or with four or more use this layout (their
+// matrix is shared by all columns); two columns (Gray-Scott's u, v with their
+// own diffusivities) keep the one-column-per-lane bodies.
+template <int K>
+constexpr bool vec_layout() {
+  return K == 1 || K >= 4;
+}
+
+// Columns per lane: 16 bytes of the inner precision, at most K.
 template <int K, typename TI>
 constexpr int vec_cols() {
-  return K < 4 ? 1 : (int)(16 / sizeof(TI)) < K ? (int)(16 / sizeof(TI)) : K;
+  return (int)(16 / sizeof(TI)) < K ? (int)(16 / sizeof(TI)) : K;
 }
 
 template <int V, typename T>
@@ -101,7 +110,7 @@ __device__ inline A vgather(int lo, int hi, A zero, F f) {
 
 template <int V>
 constexpr int vunroll() {
-  return V >= 4 ? 2 : 4;
+  return V >= 4 ? 2 : V == 2 ? 4 : 8;
 }
 
 // Lane layout: G = K / V lanes per row; lane columns j0 .. j0 + V - 1 with

@@ -340,7 +340,7 @@ def test_team_sizes_and_reduction_modes(gpu, team):
     assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
 
 
-STREAM_CASES = ["heat_cg", "heat_classic", "advdiff_batch16", "grayscott", "newmark", "wave_fp64"]
+STREAM_CASES = ["heat_cg", "heat_classic", "advdiff_batch16", "advdiff_k1", "grayscott", "newmark", "wave_fp64"]
 
 
 @pytest.mark.parametrize("case", STREAM_CASES)
@@ -369,6 +369,19 @@ def test_stream_engine_matches_oracle(gpu, case):
                 _, u, _, _ = s.run(oracle.ADVDIFF, [dt, specs[i][3]], T, states[i][:, j].copy(),
                                    krylov=oracle.BICGSTAB, tol=1e-14)
                 assert rel(got[:, j], u) < RTOL, (i, j)
+        return
+    if case == "advdiff_k1":  # one column: FP32 BiCGSTAB on the staged one-column layout
+        m = meshes.jittered_square(40, seed=8)
+        vel = (0.6, 0.8, 0.0)
+        s = oracle.System(m, bc=0, velocity=vel)
+        ctx = FEMContext(m, 0, velocity=vel)
+        sol = make_solver(ctx, Equation.Advection_Diffusion, AdvectionDiffusionParameters(1e-3, 0.05, vel),
+                          engine=_lib.FEM_ENGINE_STREAM)
+        u = s.gather(gauss(m, (0.4, 0.45), 0.02))
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        _, u, _, _ = s.run(oracle.ADVDIFF, [1e-3, 0.05], 5, u.copy(), krylov=oracle.BICGSTAB, tol=1e-14)
+        assert sol.advance_time(5) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
         return
     if case == "grayscott":
         m = meshes.icosphere(3)
