@@ -384,6 +384,26 @@ int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats) {
   })
 }
 
+int fem_batch_graph_capture(fem_batch* b, int32_t nsteps) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    return batch_graph_capture(b, nsteps);
+  })
+}
+
+int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    return batch_graph_launch(b, stats);
+  })
+}
+
 int fem_batch_sync(fem_batch* b) {
   if (!b) return FEM_ERR_INVALID;
   FEM_HIP(hipStreamSynchronize(b->stream));

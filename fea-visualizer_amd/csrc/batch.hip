@@ -441,6 +441,7 @@ int build_plan(fem_batch* b) {
   b->grid = grid;
   b->team_sizes = sizes;
   b->plan_ready = true;
+  b->plan_gen++;
   return 0;
 }
 
@@ -451,6 +452,7 @@ void batch_free(fem_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto& h : b->insts) free_inst(h);
   free_plan(b);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
   if (b->d_plain) (void)hipFree(b->d_plain);
   if (b->splan) stream_plan_free(b->splan);
   if (b->d_sinsts) (void)hipFree(b->d_sinsts);
@@ -850,6 +852,94 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   }
   if (nsteps > 0)
     for (auto& h : b->insts) h.hist = true;
+  return rc;
+}
+
+// Record one advance of nsteps (persistent engine) as a HIP graph: the
+// barrier-slot reset kernel (a fill kernel, no memset node) and the stepping
+// kernel.  Replays reuse the plan's device arrays; a plan rebuilt since
+// (new equation, new matrices, autotuning) makes the graph stale.
+int batch_graph_capture(fem_batch* b, int nsteps) {
+  if (nsteps <= 0) {
+    set_error("graph capture: nsteps must be positive");
+    return FEM_ERR_INVALID;
+  }
+  if (b->opts.engine != FEM_ENGINE_PERSISTENT) {
+    set_error("graph capture records the persistent engine's launch; the other engines are host-driven");
+    return FEM_ERR_INVALID;
+  }
+  int rc = batch_prepare(b, true);
+  if (rc) return rc;
+  if (b->graph) {
+    FEM_HIP(hipGraphExecDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  FEM_HIP(hipGraphCreate(&g, 0));
+  int ninst = (int)b->insts.size();
+  Slot* slots = b->d_slots;
+  int nslots = b->grid, nteams = b->nteams;
+  double* scal = b->d_scal;
+  int* abort_flag = b->d_abort;
+  Inst* insts = b->d_insts;
+  void* rargs[] = {(void*)&slots, (void*)&nslots, (void*)&scal, (void*)&nteams, (void*)&abort_flag, (void*)&insts,
+                   (void*)&ninst};
+  hipKernelNodeParams prm{};
+  prm.func = (void*)k_reset;
+  prm.gridDim = dim3(div_up(std::max(b->nteams + b->grid, ninst), kBlock));
+  prm.blockDim = dim3(kBlock);
+  prm.sharedMemBytes = 0;
+  prm.kernelParams = rargs;
+  prm.extra = nullptr;
+  hipGraphNode_t reset = nullptr, step = nullptr;
+  hipError_t e = hipGraphAddKernelNode(&reset, g, nullptr, 0, &prm);
+  if (e != hipSuccess) rc = hip_fail(e, "hipGraphAddKernelNode(k_reset)");
+  if (!rc) rc = step_graph_node(b->launch, nsteps, g, reset, &step);
+  if (!rc) {
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      b->graph = nullptr;
+      rc = hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  b->graph_nsteps = nsteps;
+  b->graph_gen = b->plan_gen;
+  return 0;
+}
+
+// Replay the recorded advance: the same as batch_advance(b, graph_nsteps).
+int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
+  if (!b->graph) {
+    set_error("graph launch: no graph was captured");
+    return FEM_ERR_STATE;
+  }
+  if (b->opts.engine != FEM_ENGINE_PERSISTENT) {
+    set_error("graph launch: the batch no longer runs the persistent engine; capture again");
+    return FEM_ERR_STATE;
+  }
+  int rc = batch_prepare(b, true);  // uploads the current descriptors into the plan's arrays
+  if (rc) return rc;
+  if (b->graph_gen != b->plan_gen) {
+    set_error("graph launch: the launch plan changed since the capture; capture again");
+    return FEM_ERR_STATE;
+  }
+  FEM_HIP(hipEventRecord(b->ev0, b->stream));
+  FEM_HIP(hipGraphLaunch(b->graph, b->stream));
+  FEM_HIP(hipEventRecord(b->ev1, b->stream));
+  FEM_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  int abort_flag = 0;
+  FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
+  rc = collect_stats(b, stats, ms);
+  if (abort_flag) {
+    for (auto& h : b->insts) h.hist = false;
+    set_error("device barrier wait timed out; results of this call are invalid");
+    return FEM_ERR_TIMEOUT;
+  }
+  for (auto& h : b->insts) h.hist = true;
   return rc;
 }
 

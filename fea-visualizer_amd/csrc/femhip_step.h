@@ -36,6 +36,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
 // staged pattern (one value column, Inst::SDi present).
 bool stream_vec_layout(int k);
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
+int step_graph_node(const StepLaunch& L, int nsteps, hipGraph_t g, hipGraphNode_t dep, hipGraphNode_t* out);
 
 // Per-instance host bookkeeping.
 struct InstHost {
@@ -101,6 +102,12 @@ struct fem_batch {
   int sinsts_n = 0;
   int stream_rows = 0;  // rows per block (0 = by column count)
   long long splan_key = -1;
+  // recorded advance (persistent engine): valid while the plan it was built
+  // on is current
+  long long plan_gen = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_nsteps = 0;
+  long long graph_gen = -1;
 };
 
 struct fem_solver {
@@ -118,6 +125,8 @@ int batch_clear(fem_batch* b, int inst);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
+int batch_graph_capture(fem_batch* b, int nsteps);
+int batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_surface_values(fem_batch* b, int inst, float* d_out);
 int gl_advance(fem_batch* b, int nsteps, fem_step_stats* stats);

@@ -36,6 +36,28 @@ int launch_step(const StepLaunch& L, int nsteps, hipStream_t st) {
   return 0;
 }
 
+// The same launch as a kernel node of a HIP graph, after `dep` (may be null).
+// A graph node is an ordinary launch: the plan's grid never exceeds the
+// co-resident limit the cooperative launch checks (step_max_coresident), and
+// every barrier wait of the kernel is bounded.
+int step_graph_node(const StepLaunch& L, int nsteps, hipGraph_t g, hipGraphNode_t dep, hipGraphNode_t* out) {
+  const void* fn = pick_kernel(L.k, L.inner_fp32, L.kr, L.coh);
+  if (!fn) {
+    set_error("unsupported kernel configuration (number of right-hand sides must be 1, 2, 4 or 16)");
+    return FEM_ERR_INVALID;
+  }
+  void* args[] = {(void*)&L.teams, (void*)&L.wg_team, (void*)&L.insts, (void*)&L.team_inst, (void*)&nsteps};
+  hipKernelNodeParams prm{};
+  prm.func = const_cast<void*>(fn);
+  prm.gridDim = dim3(L.grid);
+  prm.blockDim = dim3(kBlock);
+  prm.sharedMemBytes = 0;
+  prm.kernelParams = args;
+  prm.extra = nullptr;
+  FEM_HIP(hipGraphAddKernelNode(out, g, dep ? &dep : nullptr, dep ? 1 : 0, &prm));
+  return 0;
+}
+
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks) {
   const void* fn = pick_kernel(k, inner_fp32, kr, coh);
   if (!fn) return FEM_ERR_INVALID;

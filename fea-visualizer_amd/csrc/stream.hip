@@ -692,8 +692,10 @@ struct Cover {
 };
 
 // Host checks of "anything left" in the Krylov loop: every kCheckEvery-th
-// iteration.  Instances only ever leave the lists, so between checks the
-// launches cover a superset and finished blocks exit at once.
+// iteration, and every iteration from one before the count the same sweep
+// ran in the previous step (consecutive steps need similar counts).
+// Instances only ever leave the lists, so between checks the launches cover
+// a superset and finished blocks exit at once.
 constexpr int kCheckEvery = 4;
 
 int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, hipStream_t st) {
@@ -765,6 +767,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   const Cover full{true, 0, 0};
   Cover cv{};
   const bool multi = p->ninst > 1;  // one instance: its list is all of its blocks or none
+  std::vector<int> prev_its(std::max(0, max_sweeps), 0);  // iterations each sweep ran in the last step
 #define STREAM_PH(ph, cover)                                     \
   do {                                                           \
     if ((rc = launch_phase(p, a, (ph), (cover), st))) return rc; \
@@ -784,9 +787,11 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       const Cover sweep = cv;  // covers every instance of this sweep
       // one iteration done: refresh the lists (several instances) and, every
       // kCheckEvery-th time, read whether any instance still iterates
-      int since = 0;
+      int since = 0, it_run = 0;
+      const int expect = sw < (int)prev_its.size() ? prev_its[sw] : 0;
       auto next = [&](bool* stop) -> int {
-        const bool check = ++since >= kCheckEvery;
+        ++it_run;
+        const bool check = ++since >= kCheckEvery || (expect > 0 && it_run >= expect - 1);
         *stop = false;
         if (check || multi) {
           const int r = compact(p, 0, st, &cv, check);
@@ -795,6 +800,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         if (check) {
           since = 0;
           *stop = cv.nab == 0;
+          if (*stop && sw < (int)prev_its.size()) prev_its[sw] = it_run;
         }
         return 0;
       };

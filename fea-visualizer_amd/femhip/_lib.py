@@ -188,6 +188,8 @@ SIGNATURES = [
     ("fem_batch_get_state_unknowns", C.c_int, [P, I32, I32, PD]),
     ("fem_batch_advance", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_sync", C.c_int, [P]),
+    ("fem_batch_graph_capture", C.c_int, [P, I32]),
+    ("fem_batch_graph_launch", C.c_int, [P, C.POINTER(fem_step_stats)]),
     ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_operator_create", C.c_int, [P, PD, I32, I32, C.POINTER(P)]),
     ("fem_operator_destroy", C.c_int, [P]),

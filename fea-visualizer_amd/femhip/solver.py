@@ -391,6 +391,14 @@ class Batch:
     def sync(self):
         L.check(self._lib.fem_batch_sync(self.handle))
 
+    def graph_capture(self, nsteps: int):
+        """Record one advance(nsteps) of the persistent engine as a HIP graph."""
+        L.check(self._lib.fem_batch_graph_capture(self.handle, int(nsteps)))
+
+    def graph_launch(self) -> int:
+        """Replay the recorded advance; returns the step status as advance()."""
+        return L.check(self._lib.fem_batch_graph_launch(self.handle, C.byref(self.last_stats)))
+
     def autotune(self, probe_steps: int = 5):
         """Pick the workgroup budget of the team plan (state restored);
         returns (budget, best probe ms)."""

@@ -242,6 +242,16 @@ int fem_batch_set_state_unknowns(fem_batch* b, int32_t instance, int32_t field, 
 int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, double* x);
 int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats);
 int fem_batch_sync(fem_batch* b);
+/* Record one fem_batch_advance(b, nsteps) of the persistent engine as a HIP
+ * graph owned by the batch (the barrier-slot reset kernel and the stepping
+ * kernel, no memset nodes); fem_batch_graph_launch replays it and returns
+ * what fem_batch_advance would.  The state may change between replays (it is
+ * read from device memory); a rebuilt launch plan (new equation parameters or
+ * matrices, autotuning) makes the graph stale: the launch then returns
+ * FEM_ERR_STATE and the batch must be captured again.  The streaming, plain
+ * and GL-compatible engines are host-driven and are not captured. */
+int fem_batch_graph_capture(fem_batch* b, int32_t nsteps);
+int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 /* Autotuner over the number of workgroups the batch plan spreads over
  * (*chosen; 0 when the streaming engine won); state restored as for
  * fem_solver_autotune. */

@@ -297,6 +297,63 @@ def test_deterministic_bitwise(gpu):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("k,krylov", [(4, _lib.FEM_KRYLOV_CG), (16, _lib.FEM_KRYLOV_BICGSTAB)])
+def test_batch_graph_replays_match_eager(gpu, k, krylov):
+    """A recorded advance (HIP graph: slot-reset kernel + stepping kernel)
+    replayed several times, with eager work and host reads between the
+    replays, matches an identical batch advanced eagerly bit for bit after
+    every replay, and against the oracle; a rebuilt plan makes it stale."""
+    print("case graph replay", k, krylov, flush=True)
+    rng = np.random.default_rng(11)
+    specs = [(14, 1), (22, 2), (9, 3)]
+    vel = (0.4, -0.2, 0.0)
+    ctxs, systems, states = [], [], []
+    for side, seed in specs:
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+        systems.append(oracle.System(m, bc=0, velocity=vel, normalize=False))
+        states.append(rng.normal(size=(systems[-1].n, k)))
+    adv = krylov == _lib.FEM_KRYLOV_BICGSTAB
+    prm = AdvectionDiffusionParameters(1e-3, 2e-2) if adv else HeatParameters(1e-3, 2e-2)
+    eq, oprm = (oracle.ADVDIFF, [1e-3, 2e-2])
+
+    def make():
+        b = Batch(ctxs, SolverOptions(nrhs=k, krylov=krylov))
+        for i in range(len(ctxs)):
+            b.set_equation(i, prm)
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+        return b
+    g, e = make(), make()
+    g.graph_capture(2)
+    ref = [s.copy() for s in states]
+    for rep in range(3):
+        assert g.graph_launch() == _lib.FEM_OK
+        assert g.last_stats.steps == 2
+        assert e.advance(2) == _lib.FEM_OK
+        for i, s in enumerate(systems):
+            a, b_ = g.get_state_unknowns(i), e.get_state_unknowns(i)
+            assert np.array_equal(a, b_), (rep, i)
+            for j in range(k):
+                _, ref_j, _, _ = s.run(eq if adv else oracle.HEAT, oprm, 2, ref[i][:, j].copy(),
+                                       krylov=oracle.BICGSTAB if adv else oracle.CG, tol=1e-14)
+                ref[i][:, j] = ref_j
+            assert rel(a.reshape(s.n, k), ref[i]) < RTOL, (rep, i)
+    # a state write between replays is picked up by the next replay
+    for b in (g, e):
+        b.set_state_unknowns(1, _lib.FEM_FIELD_U, states[1])
+    assert g.graph_launch() == _lib.FEM_OK and e.advance(2) == _lib.FEM_OK
+    for i in range(len(ctxs)):
+        assert np.array_equal(g.get_state_unknowns(i), e.get_state_unknowns(i))
+    # new parameters rebuild the matrices and the plan: the graph is stale
+    g.set_equation(0, AdvectionDiffusionParameters(2e-3, 2e-2) if adv else HeatParameters(2e-3, 2e-2))
+    with pytest.raises(_lib.FemError):
+        g.graph_launch()
+    g.graph_capture(1)
+    assert g.graph_launch() == _lib.FEM_OK
+    g.close()
+    e.close()
+
+
 def test_batch_mixed_team_sizes(gpu):
     """One large and several tiny meshes: the plan mixes multi-workgroup teams
     with single-workgroup teams that each walk several instances."""
