@@ -59,11 +59,17 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
-def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4):
-    """Algorithmic HBM bytes of the persistent stepping kernel (int32 indices,
-    inner vectors and values of ti bytes, FP64 refinement), every vector
-    element and matrix entry touched once per phase (cache-perfect gathers)."""
-    it = nnz * (4 + ti * kv) + n * (11 * k * ti + ti * kv)     # one Krylov iteration
+def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False):
+    """Algorithmic HBM bytes of a step sequence (int32 indices, inner vectors
+    and values of ti bytes, FP64 refinement), every vector element and matrix
+    entry touched once per phase (cache-perfect gathers).  A pipelined-CG
+    iteration moves 11 vectors and the matrix once; a BiCGSTAB iteration 17
+    vectors (p: r, p_old, v_old, rh read, p, v written; s: r, v read, s, t
+    written; x: xi, p, s, t, rh read, xi, r written) and the matrix twice."""
+    if bicgstab:
+        it = 2 * nnz * (4 + ti * kv) + n * (17 * k * ti + ti * kv)
+    else:
+        it = nnz * (4 + ti * kv) + n * (11 * k * ti + ti * kv)     # one Krylov iteration
     sw = nnz * (4 + 8 * kv) + n * (3 * k * 8 + k * ti + ti * kv)  # FP64 residual
     st = nnz * (12 + 8 * kv) + n * (36 * k + 4 * kv) + n * 32 * k  # RHS + update
     return iters * it + sweeps * sw + steps * st
@@ -444,9 +450,12 @@ def run_batched_c4(args):
     st = b.last_stats
     log("batched: timed run done", wall)
     dof = 16 * tot_n
-    # iterations and sweeps are summed over instances: apply their mean to the totals
-    ni = max(1, len(ctxs))
-    ach = step_bytes(tot_n, tot_nnz, 16, 1, st.iterations / ni, st.sweeps / ni, steps) / (st.ms * 1e-3) / 1e9
+    # algorithmic bytes instance by instance: larger meshes iterate more
+    alg = 0.0
+    for i, c in enumerate(ctxs):
+        si = b.instance_stats(i)
+        alg += step_bytes(c.num_unknowns(), c.info().nnz, 16, 1, si.iterations, si.sweeps, steps, bicgstab=True)
+    ach = alg / (st.ms * 1e-3) / 1e9
     finite = all(np.all(np.isfinite(b.get_state_unknowns(i))) for i in range(len(ctxs)))
     out = contract_line(METRIC, dof * steps / wall, "DOF·steps/s", steps, 0, 1e3 * wall / steps,
                         "f32+f64 (FP32 BiCGSTAB, FP64 refinement residual and dots)",

@@ -384,6 +384,16 @@ int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats) {
   })
 }
 
+int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!b || !stats) {
+      set_error("null batch or stats");
+      return FEM_ERR_INVALID;
+    }
+    return batch_instance_stats(b, instance, stats);
+  })
+}
+
 int fem_batch_graph_capture(fem_batch* b, int32_t nsteps) {
   FEM_GUARD({
     if (!b) {

@@ -401,7 +401,9 @@ int build_plan(fem_batch* b) {
     ht[t].abort_flag = b->d_abort;
     // every workgroup reading every partial costs size^2 * nvals loads per
     // barrier; beyond ~16 KB per workgroup rank 0 reduces and publishes
-    ht[t].allpoll = ((long long)sizes[t] * 3 * k <= 2048) ? 1 : 0;
+    ht[t].allpoll = b->poll_override == 1   ? 1
+                    : b->poll_override == 2 ? 0
+                                            : ((long long)sizes[t] * 3 * k <= 2048) ? 1 : 0;
     poff += sizes[t];
   }
   std::vector<Inst> hi(ninst);
@@ -819,6 +821,34 @@ int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
   return FEM_OK;
 }
 
+int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats) {
+  if (inst < 0 || inst >= (int)b->insts.size()) {
+    set_error("instance index out of range");
+    return FEM_ERR_INVALID;
+  }
+  const InstHost& h = b->insts[inst];
+  if (!h.d_counters) {
+    set_error("instance has no state yet (set_equation first)");
+    return FEM_ERR_STATE;
+  }
+  long long cnt[4];
+  int st[4];
+  double lr[kMaxK];
+  FEM_HIP(hipMemcpyAsync(cnt, h.d_counters, sizeof(cnt), hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipMemcpyAsync(st, h.d_status, sizeof(st), hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipMemcpyAsync(lr, h.d_last_res, sizeof(lr), hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  double res = 0.0;
+  for (int j = 0; j < h.k; j++) res = std::max(res, lr[j]);
+  stats->iterations = cnt[0];
+  stats->sweeps = cnt[1];
+  stats->steps = cnt[2];
+  stats->residual = res;
+  stats->ms = 0.0;  // device time is per call, not per instance
+  stats->status = st[0];
+  return 0;
+}
+
 int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   if (nsteps < 0) {
     set_error("nsteps must be >= 0");
@@ -1040,28 +1070,36 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     return 0;
   };
   const int saved_team = b->opts.team_size, saved_budget = b->grid_budget, saved_mode = b->mode_override;
+  const int saved_poll = b->poll_override;
   double best = 1e300;
-  int best_c = single ? saved_team : saved_budget, best_m = saved_mode;
+  int best_c = single ? saved_team : saved_budget, best_m = saved_mode, best_p = saved_poll;
   if (!rc) {
     for (int c : cands) {
       for (int m : {kExchangeCoherent, kExchangeFenced}) {
         if (b->opts.exchange != FEM_EXCHANGE_AUTO && m != kExchangeCoherent) continue;  // mode fixed by opts
-        if (single) b->opts.team_size = c;
-        else b->grid_budget = c;
-        b->mode_override = m;
-        b->plan_ready = false;
-        if ((rc = restore())) break;
-        fem_step_stats st{};
-        const int r = batch_advance(b, probe, &st);
-        if (r < 0) {
-          rc = r;
-          break;
+        // one instance: both barrier reductions (all workgroups sum, or rank 0 publishes)
+        for (int pl : {1, 2}) {
+          if (!single && pl == 2) continue;
+          if (single) b->opts.team_size = c;
+          else b->grid_budget = c;
+          b->mode_override = m;
+          b->poll_override = single ? pl : saved_poll;
+          b->plan_ready = false;
+          if ((rc = restore())) break;
+          fem_step_stats st{};
+          const int r = batch_advance(b, probe, &st);
+          if (r < 0) {
+            rc = r;
+            break;
+          }
+          if (st.ms < best) {
+            best = st.ms;
+            best_c = c;
+            best_m = m;
+            best_p = b->poll_override;
+          }
         }
-        if (st.ms < best) {
-          best = st.ms;
-          best_c = c;
-          best_m = m;
-        }
+        if (rc) break;
       }
       if (rc) break;
     }
@@ -1100,6 +1138,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   if (single) b->opts.team_size = rc ? saved_team : best_c;
   else b->grid_budget = rc ? saved_budget : best_c;
   b->mode_override = rc ? saved_mode : best_m;
+  b->poll_override = rc ? saved_poll : best_p;
   b->stream_rows = (!rc && stream_best) ? best_rows : saved_rows;
   if (!rc && stream_best) {
     b->opts.engine = FEM_ENGINE_STREAM;

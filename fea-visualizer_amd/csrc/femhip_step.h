@@ -92,6 +92,8 @@ struct fem_batch {
   int nteams = 0;
   int grid_budget = 0;          // multi-instance plans: workgroups to spread over (0 = all co-resident)
   int mode_override = 0;        // exchange mode forced by the autotuner (0 = by the row-cache rule)
+  int poll_override = 0;        // barrier reduction forced by the autotuner: 1 every workgroup sums all
+                                // partials, 2 rank 0 sums and publishes (0 = by the traffic rule)
   std::vector<int> team_sizes;  // for reporting
   // plain-engine scratch
   double* d_plain = nullptr;
@@ -126,6 +128,7 @@ int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, i
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
 int batch_graph_capture(fem_batch* b, int nsteps);
+int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats);
 int batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_surface_values(fem_batch* b, int inst, float* d_out);

@@ -188,6 +188,7 @@ SIGNATURES = [
     ("fem_batch_get_state_unknowns", C.c_int, [P, I32, I32, PD]),
     ("fem_batch_advance", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_sync", C.c_int, [P]),
+    ("fem_batch_instance_stats", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_graph_capture", C.c_int, [P, I32]),
     ("fem_batch_graph_launch", C.c_int, [P, C.POINTER(fem_step_stats)]),
     ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),

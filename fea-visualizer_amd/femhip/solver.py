@@ -391,6 +391,12 @@ class Batch:
     def sync(self):
         L.check(self._lib.fem_batch_sync(self.handle))
 
+    def instance_stats(self, instance: int):
+        """Counters of one instance over the last advance (fem_step_stats)."""
+        st = L.fem_step_stats()
+        L.check(self._lib.fem_batch_instance_stats(self.handle, int(instance), C.byref(st)))
+        return st
+
     def graph_capture(self, nsteps: int):
         """Record one advance(nsteps) of the persistent engine as a HIP graph."""
         L.check(self._lib.fem_batch_graph_capture(self.handle, int(nsteps)))

@@ -242,6 +242,10 @@ int fem_batch_set_state_unknowns(fem_batch* b, int32_t instance, int32_t field, 
 int fem_batch_get_state_unknowns(fem_batch* b, int32_t instance, int32_t field, double* x);
 int fem_batch_advance(fem_batch* b, int32_t nsteps, fem_step_stats* stats);
 int fem_batch_sync(fem_batch* b);
+/* Counters of one instance over the last advance: its Krylov iterations,
+ * sweeps and steps, its final residual and outcome bits (ms is 0: device
+ * time is reported per call). */
+int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* stats);
 /* Record one fem_batch_advance(b, nsteps) of the persistent engine as a HIP
  * graph owned by the batch (the barrier-slot reset kernel and the stepping
  * kernel, no memset nodes); fem_batch_graph_launch replays it and returns

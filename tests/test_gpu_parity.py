@@ -209,6 +209,11 @@ def test_batch_uneven_meshes_multi_rhs(gpu):
         b.set_equation(i, HeatParameters(1e-3, kap))
         b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
     assert b.advance(3) == _lib.FEM_OK
+    # per-instance counters add up to the call's totals
+    per = [b.instance_stats(i) for i in range(len(specs))]
+    assert sum(p.iterations for p in per) == b.last_stats.iterations
+    assert sum(p.sweeps for p in per) == b.last_stats.sweeps
+    assert all(p.steps == 3 and p.iterations > 0 for p in per)
     for i, s in enumerate(systems):
         got = b.get_state_unknowns(i).reshape(s.n, k)
         for j in range(k):
