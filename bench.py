@@ -59,6 +59,9 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
+PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2}  # FEM_PREDICTOR_*
+
+
 def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False):
     """Algorithmic HBM bytes of a step sequence (int32 indices, inner vectors
     and values of ti bytes, FP64 refinement), every vector element and matrix
@@ -596,6 +599,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="quadratic",
+                    help="grayscott: initial guess of each solve")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
@@ -644,7 +649,8 @@ def main():
     ctx.parameters[Equation.Reaction_Diffusion] = prm
 
     def solver(**kw):
-        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol, team_size=args.team_size)
+        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol, team_size=args.team_size,
+                          predictor=PREDICTORS[args.predictor])
         for k_, v_ in kw.items():
             setattr(o, k_, v_)
         s = HIPSolver(ctx, o)

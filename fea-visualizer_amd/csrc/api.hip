@@ -64,6 +64,7 @@ void fem_opts_default(fem_solver_opts* o) {
   o->warm_start = 1;
   o->tol = 1e-10;
   o->inner_tol = 1e-4;
+  o->predictor = FEM_PREDICTOR_QUADRATIC;
 }
 
 // ------------------------------------------------------------------ context
@@ -301,7 +302,7 @@ int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr) {
     int rc = batch_field_ptr(s->batch, 0, field, &d, &col, &nc);
     if (rc) return rc;
     *dptr = (void*)d;
-    s->batch->insts[0].hist = false;  // the caller may write the state: restart the predictor
+    s->batch->insts[0].hist = 0;  // the caller may write the state: restart the predictor
     return FEM_OK;
   })
 }

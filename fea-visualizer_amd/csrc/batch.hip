@@ -263,10 +263,15 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.last_res = h.d_last_res;
   I.traj = h.traj;
   I.traj_every = h.traj ? std::max(1, h.traj_every) : 1;
-  I.extrap = (o.warm_start && o.predictor == FEM_PREDICTOR_LINEAR &&
-              (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION ||
-               h.eq == FEM_WAVE) && h.eq_override < 0) ? 1 : 0;
-  I.hist = (I.extrap && h.hist) ? 1 : 0;
+  // predictor order: the quadratic one keeps x_{n-2} in `v`, which only the
+  // equations without a velocity field leave free (the wave's `v` is its state)
+  const bool first_order = h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION;
+  I.extrap = 0;
+  if (o.warm_start && h.eq_override < 0 && (first_order || h.eq == FEM_WAVE)) {
+    if (o.predictor == FEM_PREDICTOR_LINEAR) I.extrap = 1;
+    else if (o.predictor == FEM_PREDICTOR_QUADRATIC) I.extrap = first_order ? 2 : 1;
+  }
+  I.hist = std::min(h.hist, I.extrap);
 }
 
 void free_plan(fem_batch* b) {
@@ -578,7 +583,7 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
     }
     if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;
-      h.hist = false;
+      h.hist = 0;
     }
     if (h.eq != eq) b->plan_ready = false;
     h.eq = eq;
@@ -723,7 +728,7 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
   hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
   FEM_HIP(hipStreamSynchronize(b->stream));
   (void)hipFree(tmp);
-  h.hist = false;  // the predictor's history no longer matches the state
+  h.hist = 0;  // the predictor's history no longer matches the state
   return 0;
 }
 
@@ -755,7 +760,7 @@ int batch_clear(fem_batch* b, int inst) {
     FEM_HIP(hipMemsetAsync(h.d_a, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_x, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_status, 0, sizeof(int), b->stream));
-    h.hist = false;
+    h.hist = 0;
   }
   FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
@@ -876,12 +881,12 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
   rc = collect_stats(b, stats, ms);
   if (abort_flag) {
-    for (auto& h : b->insts) h.hist = false;
+    for (auto& h : b->insts) h.hist = 0;
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
   if (nsteps > 0)
-    for (auto& h : b->insts) h.hist = true;
+    for (auto& h : b->insts) h.hist = std::min(2, h.hist + nsteps);
   return rc;
 }
 
@@ -965,11 +970,11 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
   FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
   rc = collect_stats(b, stats, ms);
   if (abort_flag) {
-    for (auto& h : b->insts) h.hist = false;
+    for (auto& h : b->insts) h.hist = 0;
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
-  for (auto& h : b->insts) h.hist = true;
+  for (auto& h : b->insts) h.hist = std::min(2, h.hist + b->graph_nsteps);
   return rc;
 }
 
@@ -1021,7 +1026,7 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   rc = collect_stats(b, stats, ms);
   if (rc >= 0 && nsteps > 0)
-    for (auto& h : b->insts) h.hist = true;
+    for (auto& h : b->insts) h.hist = std::min(2, h.hist + nsteps);
   return rc;
 }
 
@@ -1058,14 +1063,14 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
       snap.push_back(d);
     }
   }
-  std::vector<char> hist0;
-  for (auto& h : b->insts) hist0.push_back(h.hist ? 1 : 0);
+  std::vector<int> hist0;
+  for (auto& h : b->insts) hist0.push_back(h.hist);
   auto restore = [&]() -> int {
     size_t q = 0, i = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
       for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x}) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
-      h.hist = hist0[i++] != 0;
+      h.hist = hist0[i++];
     }
     return 0;
   };

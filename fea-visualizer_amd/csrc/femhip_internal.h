@@ -91,8 +91,9 @@ struct Inst {
   double* last_res;      // [k] final relative residual of the last solve
   double* traj;          // optional: primary variable after every traj_every-th
   int traj_every;        //   step of a launch, at traj[(step + 1) / traj_every - 1][n][k]
-  int extrap;            // initial guess 2 x_n - x_{n-1} (x_{n-1} kept in `a`)
-  int hist;              // `a` holds x_{n-1} at launch start
+  int extrap;            // predictor order: 0 x_n, 1 2 x_n - x_{n-1} (x_{n-1} kept in `a`),
+                         //   2 3 x_n - 3 x_{n-1} + x_{n-2} (x_{n-2} kept in `v`)
+  int hist;              // history levels held at launch start (0..extrap)
   int pad2_;
 };
 

@@ -52,7 +52,7 @@ struct InstHost {
   int eq_override = -1;      // >= 0: equation id the next launches run (EQ_SOLVE)
   double* traj = nullptr;    // trajectory record (see Inst::traj)
   int traj_every = 0;
-  bool hist = false;         // `a` holds the previous solution (predictor)
+  int hist = 0;              // predictor history levels held (x_{n-1} in `a`, x_{n-2} in `v`)
   double* d_S64 = nullptr;
   void* d_Si = nullptr;
   double* d_dinv64 = nullptr;

@@ -307,7 +307,7 @@ int gl_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
       status_all |= flag;
       rr_max = std::max(rr_max, (double)rr);
       its_all += its;
-      h.hist = false;
+      h.hist = 0;
     }
     (void)hipFree(slab);
     if (rc) return rc;

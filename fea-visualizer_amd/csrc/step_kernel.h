@@ -364,8 +364,10 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
-  // predictor: the guess 2 x_n - x_{n-1} once the previous solution is known
-  const bool ex = warm && I.extrap && (step > 0 || I.hist);
+  // predictor order this step: 2 x_n - x_{n-1} once the previous solution is
+  // known, 3 x_n - 3 x_{n-1} + x_{n-2} once two are
+  const int lv = warm ? min(I.extrap, I.hist + step) : 0;
+  const bool ex = lv > 0;
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -383,7 +385,9 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + jr;
           const double uc = ld<C>(I.u + oc);
-          const double xc = ex ? 2.0 * uc - ld<C>(I.a + oc) : uc;
+          const double xc = lv == 2 ? 3.0 * (uc - ld<C>(I.a + oc)) + ld<C>(I.v + oc)
+                            : ex    ? 2.0 * uc - ld<C>(I.a + oc)
+                                    : uc;
           return D2{M64[s] * uc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
         });
         b = g.a * inv_dt;
@@ -396,7 +400,10 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         }
         sx = g.b;
         const double uo = ld<C>(I.u + o);
-        x0 = warm ? (ex ? 2.0 * uo - ld<C>(I.a + o) : uo) : 0.0;
+        x0 = !warm     ? 0.0
+             : lv == 2 ? 3.0 * (uo - ld<C>(I.a + o)) + ld<C>(I.v + o)
+             : ex      ? 2.0 * uo - ld<C>(I.a + o)
+                       : uo;
       } break;
       case EQ_WAVE: {  // b = (M/dt) v - c^2 K u                    CPUSolver.cpp:114
         const D3 g = row_gather(lo, hi, D3{0.0, 0.0, 0.0}, [&](int s) {
@@ -922,7 +929,10 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS:
-        if (I.extrap) st<C>(I.a + o, ld<C>(I.u + o));  // x_{n-1} for the predictor
+        if (I.extrap) {  // history for the predictor: x_{n-2} <- x_{n-1} <- x_n
+          if (I.extrap == 2) st<C>(I.v + o, ld<C>(I.a + o));
+          st<C>(I.a + o, ld<C>(I.u + o));
+        }
         st<C>(I.u + o, xf);
         break;
       case EQ_WAVE: {

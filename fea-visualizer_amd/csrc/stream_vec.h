@@ -148,7 +148,8 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
-  const bool ex = warm && I.extrap && (step > 0 || I.hist);
+  const int lv = warm ? min(I.extrap, I.hist + step) : 0;  // predictor order this step
+  const bool ex = lv > 0;
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -166,7 +167,9 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
           const DV uc = vld<V>(I.u + oc);
-          const DV xc = ex ? 2.0 * uc - vld<V>(I.a + oc) : uc;
+          const DV xc = lv == 2 ? 3.0 * (uc - vld<V>(I.a + oc)) + vld<V>(I.v + oc)
+                        : ex    ? 2.0 * uc - vld<V>(I.a + oc)
+                                : uc;
           return P2<DV>{M64[s] * uc, warm ? S64[s] * xc : zd};
         });
         b = g.a * inv_dt;
@@ -180,7 +183,10 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         }
         sx = g.b;
         const DV uo = vld<V>(I.u + o);
-        x0 = warm ? (ex ? 2.0 * uo - vld<V>(I.a + o) : uo) : zd;
+        x0 = !warm     ? zd
+             : lv == 2 ? 3.0 * (uo - vld<V>(I.a + o)) + vld<V>(I.v + o)
+             : ex      ? 2.0 * uo - vld<V>(I.a + o)
+                       : uo;
       } break;
       case EQ_WAVE: {  // b = (M/dt) v - c^2 K u                    CPUSolver.cpp:114
         const P3<DV> g = vgather<U>(lo, hi, P3<DV>{zd, zd, zd}, [&](int s) {

@@ -547,6 +547,66 @@ def test_stream_engine_multi_rhs(gpu, eq, krylov, k, prec):
     b.close()
 
 
+@pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC])
+@pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream"])
+def test_predictors(gpu, case, pred):
+    """The linear (2 x_n - x_{n-1}) and quadratic (3 x_n - 3 x_{n-1} +
+    x_{n-2}, x_{n-2} kept in the v field) initial guesses change only where
+    each solve starts: results stay within the tolerance of the oracle, over
+    launches of one and of several steps (the history carries across them)."""
+    print("case predictor", case, pred, flush=True)
+    Q = pred
+    eng = _lib.FEM_ENGINE_STREAM if case.endswith("stream") else _lib.FEM_ENGINE_PERSISTENT
+    if case == "grayscott_persistent":
+        m = meshes.icosphere(3)
+        s = oracle.System(m, bc=1)
+        rng = np.random.default_rng(2)
+        u, v = np.ones(s.n), np.where(rng.uniform(size=s.n) < 0.05, 0.25, 0.0)
+        prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+        sol = make_solver(FEMContext(m, 1), Equation.Reaction_Diffusion, prm, predictor=Q, engine=eng)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        sol.set_state_unknowns(_lib.FEM_FIELD_V, v)
+        op = [0.5, 2e-4, 1e-4, 0.037, 0.06, 1]
+        for n in (1, 1, 4, 6):
+            for _ in range(n):
+                _, u, v, _ = s.step(oracle.GRAYSCOTT, op, u, v, tol=1e-14)
+            assert sol.advance_time(n) == _lib.FEM_OK
+            assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
+            assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+        return
+    if case == "advdiff16_stream":
+        k, vel = 16, (0.5, 0.3, 0.0)
+        rng = np.random.default_rng(9)
+        ctxs, systems, states = [], [], []
+        for side, seed in ((17, 1), (11, 2)):
+            m = meshes.jittered_square(side, seed=seed)
+            ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+            systems.append(oracle.System(m, bc=0, velocity=vel, normalize=False))
+            states.append(rng.normal(size=(systems[-1].n, k)))
+        b = Batch(ctxs, SolverOptions(nrhs=k, engine=eng, predictor=Q))
+        for i in range(2):
+            b.set_equation(i, AdvectionDiffusionParameters(1e-3, 3e-2))
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+        for n in (1, 5):
+            assert b.advance(n) == _lib.FEM_OK
+            for i, s in enumerate(systems):
+                for j in range(k):
+                    _, states[i][:, j], _, _ = s.run(oracle.ADVDIFF, [1e-3, 3e-2], n, states[i][:, j].copy(),
+                                                      krylov=oracle.BICGSTAB, tol=1e-14)
+                assert rel(b.get_state_unknowns(i).reshape(s.n, k), states[i]) < RTOL, (n, i)
+        b.close()
+        return
+    m = meshes.jittered_square(40, seed=8)
+    s = oracle.System(m, bc=0)
+    u = s.gather(gauss(m, (0.45, 0.5), 0.02))
+    sol = make_solver(FEMContext(m, 0), Equation.Heat, HeatParameters(1e-3, 1.0), predictor=Q, engine=eng)
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+    for n in (1, 2, 7):
+        _, u, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], n, u.copy(), tol=1e-14)
+        assert sol.advance_time(n) == _lib.FEM_OK
+        assert rel(sol.get_state_unknowns(), u) < RTOL, n
+
+
 def test_stream_engine_adjoint_matches_persistent(gpu):
     from femhip import Adjoint
     m = meshes.jittered_square(24, seed=12)
