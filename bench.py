@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # against a plain copy kernel) for the dominant kernel of a workload; the
 # summaries with their commands are in profiles/.
 PROFILED_TRAFFIC = {
-    "grayscott": {"bytes_per_step": 6.5085e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
+    "grayscott": {"bytes_per_step": 4.2973e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
     "spmv": {"bytes_per_launch": 1.2095e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
 }
 METRIC = "DOF·steps/s at rel-L2≤1e-8 vs FP64"
@@ -59,7 +59,7 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
-PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2}  # FEM_PREDICTOR_*
+PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3}  # FEM_PREDICTOR_*
 
 
 def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False):
@@ -429,6 +429,7 @@ def run_batched_c4(args):
         prms.append(AdvectionDiffusionParameters(1e-3, kap))
 
     def make(**kw):
+        kw.setdefault("predictor", PREDICTORS[args.predictor])
         b = Batch(ctxs, SolverOptions(nrhs=16, **kw))
         for i in range(len(ctxs)):
             b.set_equation(i, prms[i])
@@ -599,8 +600,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="quadratic",
-                    help="grayscott: initial guess of each solve")
+    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="cubic",
+                    help="grayscott, batched: initial guess of each solve")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")
     ap.add_argument("--inner-tol", type=float, default=1e-5)

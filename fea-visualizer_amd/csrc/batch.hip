@@ -97,7 +97,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 }
 
 void free_inst(InstHost& h) {
-  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v, h.d_a, h.d_x, h.d_b, h.d_vec,
+  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v, h.d_a, h.d_h3, h.d_x, h.d_b, h.d_vec,
                   h.d_status, h.d_counters, h.d_last_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -252,6 +252,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.u = h.d_u;
   I.v = h.d_v;
   I.a = h.d_a;
+  I.h3 = h.d_h3;
   I.x = h.d_x;
   I.b = h.d_b;
   char* base = (char*)h.d_vec;
@@ -270,6 +271,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   if (o.warm_start && h.eq_override < 0 && (first_order || h.eq == FEM_WAVE)) {
     if (o.predictor == FEM_PREDICTOR_LINEAR) I.extrap = 1;
     else if (o.predictor == FEM_PREDICTOR_QUADRATIC) I.extrap = first_order ? 2 : 1;
+    else if (o.predictor == FEM_PREDICTOR_CUBIC) I.extrap = !first_order ? 1 : h.d_h3 ? 3 : 2;
   }
   I.hist = std::min(h.hist, I.extrap);
 }
@@ -581,6 +583,14 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
       if (rc) return rc;
       b->plan_ready = false;
     }
+    // the cubic predictor's third history vector, for the equations that use it
+    if (!h.d_h3 && b->opts.predictor == FEM_PREDICTOR_CUBIC &&
+        (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)) {
+      int rc = 0;
+      h.d_h3 = dalloc<double>((size_t)h.ctx->n * k, &rc);
+      if (rc) return rc;
+      FEM_HIP(hipMemsetAsync(h.d_h3, 0, (size_t)h.ctx->n * k * sizeof(double), b->stream));
+    }
     if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;
       h.hist = 0;
@@ -886,7 +896,7 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     return FEM_ERR_TIMEOUT;
   }
   if (nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(2, h.hist + nsteps);
+    for (auto& h : b->insts) h.hist = std::min(3, h.hist + nsteps);
   return rc;
 }
 
@@ -974,7 +984,7 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
-  for (auto& h : b->insts) h.hist = std::min(2, h.hist + b->graph_nsteps);
+  for (auto& h : b->insts) h.hist = std::min(3, h.hist + b->graph_nsteps);
   return rc;
 }
 
@@ -1026,7 +1036,7 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   rc = collect_stats(b, stats, ms);
   if (rc >= 0 && nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(2, h.hist + nsteps);
+    for (auto& h : b->insts) h.hist = std::min(3, h.hist + nsteps);
   return rc;
 }
 
@@ -1056,7 +1066,8 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   std::vector<double*> snap;
   for (auto& h : b->insts) {
     const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x}) {
+    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3}) {
+      if (!src) continue;
       double* d = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (!d) break;
       FEM_HIP(hipMemcpyAsync(d, src, vb, hipMemcpyDeviceToDevice, b->stream));
@@ -1069,7 +1080,8 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     size_t q = 0, i = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x}) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
+      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3})
+        if (dst) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
       h.hist = hist0[i++];
     }
     return 0;

@@ -92,9 +92,11 @@ struct Inst {
   double* traj;          // optional: primary variable after every traj_every-th
   int traj_every;        //   step of a launch, at traj[(step + 1) / traj_every - 1][n][k]
   int extrap;            // predictor order: 0 x_n, 1 2 x_n - x_{n-1} (x_{n-1} kept in `a`),
-                         //   2 3 x_n - 3 x_{n-1} + x_{n-2} (x_{n-2} kept in `v`)
+                         //   2 3 x_n - 3 x_{n-1} + x_{n-2} (x_{n-2} kept in `v`),
+                         //   3 4 x_n - 6 x_{n-1} + 4 x_{n-2} - x_{n-3} (x_{n-3} in `h3`)
   int hist;              // history levels held at launch start (0..extrap)
   int pad2_;
+  double* h3;            // [n][k] x_{n-3} (cubic predictor only)
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

@@ -52,7 +52,9 @@ struct InstHost {
   int eq_override = -1;      // >= 0: equation id the next launches run (EQ_SOLVE)
   double* traj = nullptr;    // trajectory record (see Inst::traj)
   int traj_every = 0;
-  int hist = 0;              // predictor history levels held (x_{n-1} in `a`, x_{n-2} in `v`)
+  int hist = 0;              // predictor history levels held (x_{n-1} in `a`, x_{n-2} in `v`,
+                             // x_{n-3} in d_h3)
+  double* d_h3 = nullptr;    // cubic predictor history
   double* d_S64 = nullptr;
   void* d_Si = nullptr;
   double* d_dinv64 = nullptr;

@@ -365,7 +365,8 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
   // predictor order this step: 2 x_n - x_{n-1} once the previous solution is
-  // known, 3 x_n - 3 x_{n-1} + x_{n-2} once two are
+  // known, 3 x_n - 3 x_{n-1} + x_{n-2} once two are, 4 x_n - 6 x_{n-1} +
+  // 4 x_{n-2} - x_{n-3} once three are
   const int lv = warm ? min(I.extrap, I.hist + step) : 0;
   const bool ex = lv > 0;
   const int GA* __restrict__ rp = gp(I.rp);
@@ -385,9 +386,10 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + jr;
           const double uc = ld<C>(I.u + oc);
-          const double xc = lv == 2 ? 3.0 * (uc - ld<C>(I.a + oc)) + ld<C>(I.v + oc)
-                            : ex    ? 2.0 * uc - ld<C>(I.a + oc)
-                                    : uc;
+          const double xc = lv == 3   ? 4.0 * (uc + ld<C>(I.v + oc)) - 6.0 * ld<C>(I.a + oc) - ld<C>(I.h3 + oc)
+                            : lv == 2 ? 3.0 * (uc - ld<C>(I.a + oc)) + ld<C>(I.v + oc)
+                            : ex      ? 2.0 * uc - ld<C>(I.a + oc)
+                                      : uc;
           return D2{M64[s] * uc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
         });
         b = g.a * inv_dt;
@@ -401,6 +403,7 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         sx = g.b;
         const double uo = ld<C>(I.u + o);
         x0 = !warm     ? 0.0
+             : lv == 3 ? 4.0 * (uo + ld<C>(I.v + o)) - 6.0 * ld<C>(I.a + o) - ld<C>(I.h3 + o)
              : lv == 2 ? 3.0 * (uo - ld<C>(I.a + o)) + ld<C>(I.v + o)
              : ex      ? 2.0 * uo - ld<C>(I.a + o)
                        : uo;
@@ -929,8 +932,9 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS:
-        if (I.extrap) {  // history for the predictor: x_{n-2} <- x_{n-1} <- x_n
-          if (I.extrap == 2) st<C>(I.v + o, ld<C>(I.a + o));
+        if (I.extrap) {  // history for the predictor: x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n
+          if (I.extrap == 3) st<C>(I.h3 + o, ld<C>(I.v + o));
+          if (I.extrap >= 2) st<C>(I.v + o, ld<C>(I.a + o));
           st<C>(I.a + o, ld<C>(I.u + o));
         }
         st<C>(I.u + o, xf);

@@ -167,9 +167,10 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
           const DV uc = vld<V>(I.u + oc);
-          const DV xc = lv == 2 ? 3.0 * (uc - vld<V>(I.a + oc)) + vld<V>(I.v + oc)
-                        : ex    ? 2.0 * uc - vld<V>(I.a + oc)
-                                : uc;
+          const DV xc = lv == 3   ? 4.0 * (uc + vld<V>(I.v + oc)) - 6.0 * vld<V>(I.a + oc) - vld<V>(I.h3 + oc)
+                        : lv == 2 ? 3.0 * (uc - vld<V>(I.a + oc)) + vld<V>(I.v + oc)
+                        : ex      ? 2.0 * uc - vld<V>(I.a + oc)
+                                  : uc;
           return P2<DV>{M64[s] * uc, warm ? S64[s] * xc : zd};
         });
         b = g.a * inv_dt;
@@ -184,6 +185,7 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         sx = g.b;
         const DV uo = vld<V>(I.u + o);
         x0 = !warm     ? zd
+             : lv == 3 ? 4.0 * (uo + vld<V>(I.v + o)) - 6.0 * vld<V>(I.a + o) - vld<V>(I.h3 + o)
              : lv == 2 ? 3.0 * (uo - vld<V>(I.a + o)) + vld<V>(I.v + o)
              : ex      ? 2.0 * uo - vld<V>(I.a + o)
                        : uo;

@@ -125,14 +125,17 @@ typedef struct fem_solver_opts {
   double inner_tol;      /* FP32 sweep reduction; 0 = 1e-4 */
   int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or
                             autotuner), _COHERENT or _FENCED */
-  int32_t predictor;     /* initial guess with warm_start: FEM_PREDICTOR_QUADRATIC (the
-                            fem_opts_default choice: 3 x_n - 3 x_{n-1} + x_{n-2} for heat,
-                            advection-diffusion and Gray-Scott, linear for the wave equation),
+  int32_t predictor;     /* initial guess with warm_start, from the previous solutions:
+                            FEM_PREDICTOR_CUBIC (the fem_opts_default choice: 4 x_n -
+                            6 x_{n-1} + 4 x_{n-2} - x_{n-3}; one more state-sized history
+                            vector), FEM_PREDICTOR_QUADRATIC (3 x_n - 3 x_{n-1} + x_{n-2}),
                             FEM_PREDICTOR_LINEAR (0: 2 x_n - x_{n-1}) or
-                            FEM_PREDICTOR_PREVIOUS (x_n) */
+                            FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
+                            Gray-Scott take any order; the wave equation is linear at most,
+                            Newmark uses its acceleration. */
 } fem_solver_opts;
 
-enum { FEM_PREDICTOR_LINEAR = 0, FEM_PREDICTOR_PREVIOUS = 1, FEM_PREDICTOR_QUADRATIC = 2 };
+enum { FEM_PREDICTOR_LINEAR = 0, FEM_PREDICTOR_PREVIOUS = 1, FEM_PREDICTOR_QUADRATIC = 2, FEM_PREDICTOR_CUBIC = 3 };
 
 /* How the workgroups of one team exchange vectors between barriers:
  * COHERENT: device-coherent loads/stores, fence-free barriers (latency-bound
