@@ -97,7 +97,8 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 }
 
 void free_inst(InstHost& h) {
-  void* ptrs[] = {h.d_S64, h.d_Si, h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v, h.d_a, h.d_h3, h.d_x, h.d_b, h.d_vec,
+  void* ptrs[] = {h.d_S64, h.d_Si,    h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v,   h.d_a,
+                  h.d_h3,  h.d_xnext, h.d_hpre,   h.d_x,     h.d_b,   h.d_vec,
                   h.d_status, h.d_counters, h.d_last_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -253,6 +254,8 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.v = h.d_v;
   I.a = h.d_a;
   I.h3 = h.d_h3;
+  I.xnext = h.d_xnext;
+  I.hpre = h.d_hpre;
   I.x = h.d_x;
   I.b = h.d_b;
   char* base = (char*)h.d_vec;
@@ -782,6 +785,15 @@ int batch_prepare(fem_batch* b, bool need_plan) {
     if (h.eq < 0 || !h.d_u) {
       set_error("advance: set_equation was not called for every instance");
       return FEM_ERR_STATE;
+    }
+    // persistent plans of first-order equations fuse each update with the
+    // next right-hand side: two more state-sized vectors (Inst::xnext, hpre)
+    if (need_plan && !h.d_xnext &&
+        (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION)) {
+      int rc = 0;
+      h.d_xnext = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (!rc) h.d_hpre = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (rc) return rc;
     }
     if (h.dirty || h.formed_version != h.ctx->version) {
       int rc = form_inst(b, h);

@@ -97,6 +97,8 @@ struct Inst {
   int hist;              // history levels held at launch start (0..extrap)
   int pad2_;
   double* h3;            // [n][k] x_{n-3} (cubic predictor only)
+  double* xnext;         // [n][k] next initial guess formed with the update (persistent kernel)
+  double* hpre;          // [n][k] history part of the next initial residual
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

@@ -340,20 +340,23 @@ struct D3 {
   }
 };
 
-// sum over s in [lo, hi) of f(s), in ascending s
-template <typename A, typename F>
+// sum over s in [lo, hi) of f(s), in ascending s; U entries in flight
+template <int U = kU, typename A, typename F>
 __device__ inline A row_gather(int lo, int hi, A zero, F f) {
   A acc = zero;
-  for (int s0 = lo; s0 < hi; s0 += kU) {
-    A t[kU];
+  for (int s0 = lo; s0 < hi; s0 += U) {
+    A t[U];
 #pragma unroll
-    for (int u = 0; u < kU; u++) t[u] = f(min(s0 + u, hi - 1));
+    for (int u = 0; u < U; u++) t[u] = f(min(s0 + u, hi - 1));
 #pragma unroll
-    for (int u = 0; u < kU; u++)
+    for (int u = 0; u < U; u++)
       if (s0 + u < hi) acc_add(acc, t[u]);
   }
   return acc;
 }
+
+// Entries in flight for gathers that load five or more values per entry.
+constexpr int kUWide = 4;
 
 template <int K, typename TI, int C>
 __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step, double (&acc)[3]) {
@@ -454,9 +457,24 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
   }
 }
 
-// FP64 refinement residual r = b - S64 (x + xi).
+// Predictor weights on (x_n, x_{n-1}, x_{n-2}, x_{n-3}) for order lv:
+// 0 (1), 1 (2, -1), 2 (3, -3, 1), 3 (4, -6, 4, -1).
+__device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, double& w3) {
+  w0 = lv + 1.0;
+  w1 = lv == 3 ? -6.0 : lv == 2 ? -3.0 : lv == 1 ? -1.0 : 0.0;
+  w2 = lv == 3 ? 4.0 : lv == 2 ? 1.0 : 0.0;
+  w3 = lv == 3 ? -1.0 : 0.0;
+}
+
+// FP64 refinement residual r = b - S64 (x + xi).  With hpre, also the
+// history part S64 (w1 u + w2 a + w3 v) of the next step's initial residual
+// for predictor order lvn (the state update of this step and the right-hand
+// side of the next run as one phase, phase_update_rhs, which cannot gather
+// history its neighbours are shifting).
 template <int K, typename TI, int C>
-__device__ void phase_res(const Inst& I, int row_lo, int row_hi, TI* r, const Shared& sh, double (&acc)[2]) {
+__device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, int row_lo, int row_hi, TI* r,
+                                                                const Shared& sh, double (&acc)[2], int lvn = 0,
+                                                                double* hpre = nullptr) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -467,13 +485,29 @@ __device__ void phase_res(const Inst& I, int row_lo, int row_hi, TI* r, const Sh
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* S64 = gp(I.S64);
+  double w0, w1, w2, w3;
+  pred_weights(lvn, w0, w1, w2, w3);
+  const bool want_h = hpre && lvn > 0;
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
     const double bo = gp(I.b)[o];
-    const double sx = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
-      const size_t oc = (size_t)ci[s] * K + jr;
-      return S64[(size_t)s * kv + vj] * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc)));
-    });
+    double sx;
+    if (want_h) {
+      const D2 g = row_gather<kUWide>(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
+        const size_t oc = (size_t)ci[s] * K + jr;
+        const double sv = S64[(size_t)s * kv + vj];
+        const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
+                          (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0);
+        return D2{sv * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc))), sv * hv};
+      });
+      sx = g.a;
+      gp(hpre)[o] = g.b;
+    } else {
+      sx = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
+        const size_t oc = (size_t)ci[s] * K + jr;
+        return S64[(size_t)s * kv + vj] * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc)));
+      });
+    }
     const double r64 = bo - sx;
     const TI rr = to_ti<TI>(r64);
     st<C>(r + o, rr);
@@ -970,6 +1004,115 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
   if (f) atomicOr(flags, f);
 }
 
+// The state update of step `step` and the right-hand side of step + 1 as one
+// phase (heat, advection-diffusion, Gray-Scott: the new state is the
+// solution).  Neighbours are gathered only through x + xi, which no
+// workgroup writes in this phase; everything written belongs to the
+// workgroup's own rows: the state and predictor history, b, r, and the next
+// initial guess in xnext, which phase_install moves into x (xi = 0) after the
+// barrier.  The history part of S x0 comes from hpre (phase_res or
+// phase_hpre of this step); lvn is the next step's predictor order.
+template <int K, typename TI, int C>
+__device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, const Shared& sh, int* flags, int step,
+                                 int lvn, double (&acc)[3]) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  const TI* xi = (const TI*)I.xi;
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const double inv_dt = 1.0 / I.dt;
+  const bool warm = I.warm != 0;
+  double w0, w1, w2, w3;
+  pred_weights(lvn, w0, w1, w2, w3);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  const double GA* M64 = gp(I.M64);
+  const double GA* S64 = gp(I.S64);
+  int f = 0;
+  double* traj = nullptr;  // trajectory record of this step, if any
+  if (I.traj && (step + 1) % I.traj_every == 0)
+    traj = I.traj + (size_t)((step + 1) / I.traj_every - 1) * I.n * K;
+  // this step's solution of column j at offset idx (Eigen: rhs == 0 -> x = 0)
+  auto xn_at = [&](size_t idx, int j) -> double {
+    return sh.bb[j] == 0.0 ? 0.0 : ld<C>(I.x + idx) + (double)ld<C>(xi + idx);
+  };
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    const double xn = xn_at(o, jr);
+    // ---- update: state and history x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n (own row)
+    const double uo = ld<C>(I.u + o);
+    const double ao = I.extrap >= 2 ? ld<C>(I.a + o) : 0.0;
+    const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
+    if (I.extrap >= 3) st<C>(I.h3 + o, vo);
+    if (I.extrap >= 2) st<C>(I.v + o, ao);
+    if (I.extrap >= 1) st<C>(I.a + o, uo);
+    st<C>(I.u + o, xn);
+    if (traj) gp(traj)[o] = xn;
+    if (xn > 1e4) f |= ST_UNSTABLE;
+    if (!isfinite(xn)) f |= ST_NONFINITE;
+    // ---- next right-hand side: b = (M/dt) x_n (+ Gray-Scott reaction), r = b - S x0
+    const D2 g = row_gather<kUWide>(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
+      const double xc = xn_at((size_t)ci[s] * K + jr, jr);
+      return D2{M64[s] * xc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
+    });
+    double b = g.a * inv_dt;
+    if (I.eq == EQ_GS) {  // CPUSolver.cpp:134,137 with the new state
+      const double uu = xn_at((size_t)row * K, 0), vv = xn_at((size_t)row * K + 1, 1);
+      const double uvv = uu * (vv * vv);
+      const double fr = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
+      const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
+      b += w * fr;
+    }
+    const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo : 0.0;
+    const double sx = warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0;
+    const double r64 = b - sx;
+    gp(I.b)[o] = b;
+    gp(I.xnext)[o] = x0;
+    const TI rr = to_ti<TI>(r64);
+    st<C>(r + o, rr);
+    acc[0] += b * b;
+    acc[1] += r64 * r64;
+    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
+  }
+  if (f) atomicOr(flags, f);
+}
+
+// Own rows: x = xnext, xi = 0 (the initial guess phase_update_rhs prepared).
+template <int K, typename TI, int C>
+__device__ void phase_install(const Inst& I, int row_lo, int row_hi) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  TI* xi = (TI*)I.xi;
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    st<C>(I.x + o, gp(I.xnext)[o]);
+    st<C>(xi + o, (TI)0);
+  }
+}
+
+// The history part of the next step's initial residual (as phase_res does)
+// for a step that ran no refinement sweep.
+template <int K, typename TI, int C>
+__device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
+  double w0, w1, w2, w3;
+  pred_weights(lvn, w0, w1, w2, w3);
+  const int GA* __restrict__ rp = gp(I.rp);
+  const int GA* __restrict__ ci = gp(I.ci);
+  const double GA* S64 = gp(I.S64);
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    gp(I.hpre)[o] = row_gather<kUWide>(rp[row], rp[row + 1], 0.0, [&](int s) {
+      const size_t oc = (size_t)ci[s] * K + jr;
+      const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
+                        (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0);
+      return S64[(size_t)s * kv + vj] * hv;
+    });
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Krylov drivers (uniform control flow across the team)
 // ---------------------------------------------------------------------------
@@ -1183,11 +1326,24 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
   long long its_total = 0, sweeps_total = 0;
   int status = 0;
   TI* r0 = (TI*)I.r;
+  // first-order equations: the update of one step and the right-hand side of
+  // the next share a phase (phase_update_rhs); have_rhs: this step's
+  // right-hand side is already formed and reduced (sh.red)
+  // (each barrier has one call site: inlined copies of the wait loops spill)
+  const bool fuse_eq = (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF || I.eq == EQ_GS) && I.xnext && I.hpre;
+  bool have_rhs = false;  // the previous step's update is pending: it runs with this step's right-hand side
   for (int step = 0; step < nsteps; step++) {
-    // ---- R: right-hand side and initial residual
+    const bool fuse_next = fuse_eq && step + 1 < nsteps;
+    const int lvn = (fuse_next && I.warm) ? min(I.extrap, I.hist + step + 1) : 0;  // next step's predictor order
+    // ---- R: right-hand side and initial residual (after the previous step's
+    // update, in the same phase, when that was deferred)
     {
       double acc[3] = {0.0, 0.0, 0.0};
-      phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
+      if (have_rhs)
+        phase_update_rhs<K, TI, C>(I, row_lo, row_hi, r0, sh, I.status, step - 1,
+                                   I.warm ? min(I.extrap, I.hist + step) : 0, acc);
+      else
+        phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
     }
     if (threadIdx.x < K) {
@@ -1200,6 +1356,8 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       sh.reset[threadIdx.x] = 0;
     }
     __syncthreads();
+    // own rows only: the phases up to the next barrier read no other row's x
+    if (have_rhs) phase_install<K, TI, C>(I, row_lo, row_hi);
     int sweeps = 0, its = 0;
     while (true) {
       if (threadIdx.x == 0) {
@@ -1227,7 +1385,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       // correction and takes the residual of x alone.
       for (int pass = 0; pass < 2; pass++) {
         double acc[2] = {0.0, 0.0};
-        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc);
+        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc, lvn, fuse_next ? I.hpre : nullptr);
         if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
         if (threadIdx.x == 0) {
           int redo = 0;
@@ -1255,15 +1413,26 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     int notconv = 0;
     for (int j = 0; j < K; j++) notconv |= sh.active[j];
     if (notconv) status |= ST_NOT_CONVERGED;
-    phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status, step);
     if (rank == 0 && threadIdx.x < K) {
       const double bb = sh.bb[threadIdx.x];
       I.last_res[threadIdx.x] = bb > 0.0 ? sqrt(sh.ithr[threadIdx.x] / bb) : 0.0;
     }
     its_total += its;
     sweeps_total += sweeps;
-    // the next step (or the next instance of this team) gathers the new state
-    if (!team_barrier<K, C>(T, rank, e, sh)) return false;
+    bool barrier;
+    if (fuse_next) {
+      // ---- U deferred: it runs with the next step's right-hand side.  With
+      // no FP64 residual this step, form the history term here and share the
+      // installed x before neighbours gather it.
+      if (sweeps == 0 && lvn > 0) phase_hpre<K, TI, C>(I, row_lo, row_hi, lvn);
+      barrier = sweeps == 0 && have_rhs;
+      have_rhs = true;
+    } else {
+      phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status, step);
+      barrier = true;  // the next step (or the next instance of this team) gathers the new state
+      have_rhs = false;
+    }
+    if (barrier && !team_barrier<K, C>(T, rank, e, sh)) return false;
   }
   if (rank == 0 && threadIdx.x == 0) {
     I.counters[0] += its_total;
