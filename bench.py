@@ -59,7 +59,7 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
-PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3}  # FEM_PREDICTOR_*
+PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3, "quartic": 4}  # FEM_PREDICTOR_*
 
 
 def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False):

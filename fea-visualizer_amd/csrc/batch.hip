@@ -98,7 +98,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 
 void free_inst(InstHost& h) {
   void* ptrs[] = {h.d_S64, h.d_Si,    h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v,   h.d_a,
-                  h.d_h3,  h.d_xnext, h.d_hpre,   h.d_x,     h.d_b,   h.d_vec,
+                  h.d_h3,  h.d_h4,    h.d_xnext,  h.d_hpre,  h.d_x,   h.d_b, h.d_vec,
                   h.d_status, h.d_counters, h.d_last_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -254,6 +254,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.v = h.d_v;
   I.a = h.d_a;
   I.h3 = h.d_h3;
+  I.h4 = h.d_h4;
   I.xnext = h.d_xnext;
   I.hpre = h.d_hpre;
   I.x = h.d_x;
@@ -275,6 +276,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
     if (o.predictor == FEM_PREDICTOR_LINEAR) I.extrap = 1;
     else if (o.predictor == FEM_PREDICTOR_QUADRATIC) I.extrap = first_order ? 2 : 1;
     else if (o.predictor == FEM_PREDICTOR_CUBIC) I.extrap = !first_order ? 1 : h.d_h3 ? 3 : 2;
+    else if (o.predictor == FEM_PREDICTOR_QUARTIC) I.extrap = !first_order ? 1 : h.d_h4 ? 4 : h.d_h3 ? 3 : 2;
   }
   I.hist = std::min(h.hist, I.extrap);
 }
@@ -587,12 +589,15 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
       b->plan_ready = false;
     }
     // the cubic predictor's third history vector, for the equations that use it
-    if (!h.d_h3 && b->opts.predictor == FEM_PREDICTOR_CUBIC &&
-        (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)) {
-      int rc = 0;
-      h.d_h3 = dalloc<double>((size_t)h.ctx->n * k, &rc);
-      if (rc) return rc;
-      FEM_HIP(hipMemsetAsync(h.d_h3, 0, (size_t)h.ctx->n * k * sizeof(double), b->stream));
+    const bool hi_order = b->opts.predictor == FEM_PREDICTOR_CUBIC || b->opts.predictor == FEM_PREDICTOR_QUARTIC;
+    if (hi_order && (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)) {
+      for (double** hp : {&h.d_h3, &h.d_h4}) {
+        if (*hp || (hp == &h.d_h4 && b->opts.predictor != FEM_PREDICTOR_QUARTIC)) continue;
+        int rc = 0;
+        *hp = dalloc<double>((size_t)h.ctx->n * k, &rc);
+        if (rc) return rc;
+        FEM_HIP(hipMemsetAsync(*hp, 0, (size_t)h.ctx->n * k * sizeof(double), b->stream));
+      }
     }
     if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;
@@ -908,7 +913,7 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     return FEM_ERR_TIMEOUT;
   }
   if (nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(3, h.hist + nsteps);
+    for (auto& h : b->insts) h.hist = std::min(4, h.hist + nsteps);
   return rc;
 }
 
@@ -996,7 +1001,7 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
-  for (auto& h : b->insts) h.hist = std::min(3, h.hist + b->graph_nsteps);
+  for (auto& h : b->insts) h.hist = std::min(4, h.hist + b->graph_nsteps);
   return rc;
 }
 
@@ -1048,7 +1053,7 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   rc = collect_stats(b, stats, ms);
   if (rc >= 0 && nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(3, h.hist + nsteps);
+    for (auto& h : b->insts) h.hist = std::min(4, h.hist + nsteps);
   return rc;
 }
 
@@ -1078,7 +1083,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   std::vector<double*> snap;
   for (auto& h : b->insts) {
     const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3}) {
+    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4}) {
       if (!src) continue;
       double* d = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (!d) break;
@@ -1092,7 +1097,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     size_t q = 0, i = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3})
+      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4})
         if (dst) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
       h.hist = hist0[i++];
     }

@@ -96,7 +96,8 @@ struct Inst {
                          //   3 4 x_n - 6 x_{n-1} + 4 x_{n-2} - x_{n-3} (x_{n-3} in `h3`)
   int hist;              // history levels held at launch start (0..extrap)
   int pad2_;
-  double* h3;            // [n][k] x_{n-3} (cubic predictor only)
+  double* h3;            // [n][k] x_{n-3} (cubic and quartic predictors)
+  double* h4;            // [n][k] x_{n-4} (quartic predictor: 5 x_n - 10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4})
   double* xnext;         // [n][k] next initial guess formed with the update (persistent kernel)
   double* hpre;          // [n][k] history part of the next initial residual
 };

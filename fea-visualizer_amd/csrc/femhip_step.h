@@ -54,7 +54,8 @@ struct InstHost {
   int traj_every = 0;
   int hist = 0;              // predictor history levels held (x_{n-1} in `a`, x_{n-2} in `v`,
                              // x_{n-3} in d_h3)
-  double* d_h3 = nullptr;    // cubic predictor history
+  double* d_h3 = nullptr;    // cubic / quartic predictor history
+  double* d_h4 = nullptr;
   double* d_xnext = nullptr; // Inst::xnext, Inst::hpre
   double* d_hpre = nullptr;
   double* d_S64 = nullptr;

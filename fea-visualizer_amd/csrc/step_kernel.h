@@ -389,7 +389,9 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + jr;
           const double uc = ld<C>(I.u + oc);
-          const double xc = lv == 3   ? 4.0 * (uc + ld<C>(I.v + oc)) - 6.0 * ld<C>(I.a + oc) - ld<C>(I.h3 + oc)
+          const double xc = lv == 4   ? 5.0 * (uc - ld<C>(I.h3 + oc)) + 10.0 * (ld<C>(I.v + oc) - ld<C>(I.a + oc)) +
+                                            ld<C>(I.h4 + oc)
+                            : lv == 3 ? 4.0 * (uc + ld<C>(I.v + oc)) - 6.0 * ld<C>(I.a + oc) - ld<C>(I.h3 + oc)
                             : lv == 2 ? 3.0 * (uc - ld<C>(I.a + oc)) + ld<C>(I.v + oc)
                             : ex      ? 2.0 * uc - ld<C>(I.a + oc)
                                       : uc;
@@ -406,6 +408,7 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         sx = g.b;
         const double uo = ld<C>(I.u + o);
         x0 = !warm     ? 0.0
+             : lv == 4 ? 5.0 * (uo - ld<C>(I.h3 + o)) + 10.0 * (ld<C>(I.v + o) - ld<C>(I.a + o)) + ld<C>(I.h4 + o)
              : lv == 3 ? 4.0 * (uo + ld<C>(I.v + o)) - 6.0 * ld<C>(I.a + o) - ld<C>(I.h3 + o)
              : lv == 2 ? 3.0 * (uo - ld<C>(I.a + o)) + ld<C>(I.v + o)
              : ex      ? 2.0 * uo - ld<C>(I.a + o)
@@ -457,17 +460,19 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
   }
 }
 
-// Predictor weights on (x_n, x_{n-1}, x_{n-2}, x_{n-3}) for order lv:
-// 0 (1), 1 (2, -1), 2 (3, -3, 1), 3 (4, -6, 4, -1).
-__device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, double& w3) {
+// Predictor weights on (x_n, x_{n-1}, ..., x_{n-4}) for order lv: the
+// alternating binomial coefficients of row lv + 1, 0 (1), 1 (2, -1),
+// 2 (3, -3, 1), 3 (4, -6, 4, -1), 4 (5, -10, 10, -5, 1).
+__device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, double& w3, double& w4) {
   w0 = lv + 1.0;
-  w1 = lv == 3 ? -6.0 : lv == 2 ? -3.0 : lv == 1 ? -1.0 : 0.0;
-  w2 = lv == 3 ? 4.0 : lv == 2 ? 1.0 : 0.0;
-  w3 = lv == 3 ? -1.0 : 0.0;
+  w1 = lv == 4 ? -10.0 : lv == 3 ? -6.0 : lv == 2 ? -3.0 : lv == 1 ? -1.0 : 0.0;
+  w2 = lv == 4 ? 10.0 : lv == 3 ? 4.0 : lv == 2 ? 1.0 : 0.0;
+  w3 = lv == 4 ? -5.0 : lv == 3 ? -1.0 : 0.0;
+  w4 = lv == 4 ? 1.0 : 0.0;
 }
 
 // FP64 refinement residual r = b - S64 (x + xi).  With hpre, also the
-// history part S64 (w1 u + w2 a + w3 v) of the next step's initial residual
+// history part S64 (w1 u + w2 a + w3 v + w4 h3) of the next step's initial residual
 // for predictor order lvn (the state update of this step and the right-hand
 // side of the next run as one phase, phase_update_rhs, which cannot gather
 // history its neighbours are shifting).
@@ -485,8 +490,8 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* S64 = gp(I.S64);
-  double w0, w1, w2, w3;
-  pred_weights(lvn, w0, w1, w2, w3);
+  double w0, w1, w2, w3, w4;
+  pred_weights(lvn, w0, w1, w2, w3, w4);
   const bool want_h = hpre && lvn > 0;
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
@@ -497,7 +502,7 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
         const size_t oc = (size_t)ci[s] * K + jr;
         const double sv = S64[(size_t)s * kv + vj];
         const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
-                          (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0);
+                          (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0) + (lvn >= 4 ? w4 * ld<C>(I.h3 + oc) : 0.0);
         return D2{sv * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc))), sv * hv};
       });
       sx = g.a;
@@ -966,8 +971,9 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS:
-        if (I.extrap) {  // history for the predictor: x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n
-          if (I.extrap == 3) st<C>(I.h3 + o, ld<C>(I.v + o));
+        if (I.extrap) {  // history for the predictor: x_{n-4} <- x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n
+          if (I.extrap >= 4) st<C>(I.h4 + o, ld<C>(I.h3 + o));
+          if (I.extrap >= 3) st<C>(I.h3 + o, ld<C>(I.v + o));
           if (I.extrap >= 2) st<C>(I.v + o, ld<C>(I.a + o));
           st<C>(I.a + o, ld<C>(I.u + o));
         }
@@ -1022,8 +1028,8 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
-  double w0, w1, w2, w3;
-  pred_weights(lvn, w0, w1, w2, w3);
+  double w0, w1, w2, w3, w4;
+  pred_weights(lvn, w0, w1, w2, w3, w4);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -1043,6 +1049,8 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
     const double uo = ld<C>(I.u + o);
     const double ao = I.extrap >= 2 ? ld<C>(I.a + o) : 0.0;
     const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
+    const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
+    if (I.extrap >= 4) st<C>(I.h4 + o, ho);
     if (I.extrap >= 3) st<C>(I.h3 + o, vo);
     if (I.extrap >= 2) st<C>(I.v + o, ao);
     if (I.extrap >= 1) st<C>(I.a + o, uo);
@@ -1063,7 +1071,7 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
       const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
       b += w * fr;
     }
-    const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo : 0.0;
+    const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo + w4 * ho : 0.0;
     const double sx = warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0;
     const double r64 = b - sx;
     gp(I.b)[o] = b;
@@ -1097,8 +1105,8 @@ __device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
-  double w0, w1, w2, w3;
-  pred_weights(lvn, w0, w1, w2, w3);
+  double w0, w1, w2, w3, w4;
+  pred_weights(lvn, w0, w1, w2, w3, w4);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* S64 = gp(I.S64);
@@ -1107,7 +1115,7 @@ __device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn) {
     gp(I.hpre)[o] = row_gather<kUWide>(rp[row], rp[row + 1], 0.0, [&](int s) {
       const size_t oc = (size_t)ci[s] * K + jr;
       const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
-                        (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0);
+                        (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0) + (lvn >= 4 ? w4 * ld<C>(I.h3 + oc) : 0.0);
       return S64[(size_t)s * kv + vj] * hv;
     });
   }

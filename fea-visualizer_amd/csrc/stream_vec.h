@@ -124,9 +124,9 @@ struct VLay {
 
 // A row block's pattern staged in LDS: row pointers relative to the block's
 // first nonzero, column indices and the column-scaled inner matrix entries
-// S_ij D^-1_jj (classic CG's phase A: S_ij).  The Krylov gathers then read only the vectors from global
-// memory: one dependent memory level instead of three (row pointer -> column
-// index -> vector).
+// S_ij D^-1_jj (classic CG's phase A: S_ij).  The Krylov gathers then read
+// only the vectors from global memory: one dependent memory level instead of
+// three (row pointer -> column index -> vector).
 #define LA __attribute__((address_space(3)))
 template <typename TI>
 struct Stage {
@@ -167,7 +167,9 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
           const DV uc = vld<V>(I.u + oc);
-          const DV xc = lv == 3   ? 4.0 * (uc + vld<V>(I.v + oc)) - 6.0 * vld<V>(I.a + oc) - vld<V>(I.h3 + oc)
+          const DV xc = lv == 4   ? 5.0 * (uc - vld<V>(I.h3 + oc)) + 10.0 * (vld<V>(I.v + oc) - vld<V>(I.a + oc)) +
+                                        vld<V>(I.h4 + oc)
+                        : lv == 3 ? 4.0 * (uc + vld<V>(I.v + oc)) - 6.0 * vld<V>(I.a + oc) - vld<V>(I.h3 + oc)
                         : lv == 2 ? 3.0 * (uc - vld<V>(I.a + oc)) + vld<V>(I.v + oc)
                         : ex      ? 2.0 * uc - vld<V>(I.a + oc)
                                   : uc;
@@ -185,6 +187,7 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         sx = g.b;
         const DV uo = vld<V>(I.u + o);
         x0 = !warm     ? zd
+             : lv == 4 ? 5.0 * (uo - vld<V>(I.h3 + o)) + 10.0 * (vld<V>(I.v + o) - vld<V>(I.a + o)) + vld<V>(I.h4 + o)
              : lv == 3 ? 4.0 * (uo + vld<V>(I.v + o)) - 6.0 * vld<V>(I.a + o) - vld<V>(I.h3 + o)
              : lv == 2 ? 3.0 * (uo - vld<V>(I.a + o)) + vld<V>(I.v + o)
              : ex      ? 2.0 * uo - vld<V>(I.a + o)

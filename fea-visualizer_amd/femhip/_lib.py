@@ -58,6 +58,7 @@ FEM_PREDICTOR_LINEAR = 0
 FEM_PREDICTOR_PREVIOUS = 1
 FEM_PREDICTOR_QUADRATIC = 2
 FEM_PREDICTOR_CUBIC = 3
+FEM_PREDICTOR_QUARTIC = 4
 FEM_SPMV_STREAM = 32
 FEM_SPMV_STREAM_NT = 33
 SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)

@@ -129,13 +129,20 @@ typedef struct fem_solver_opts {
                             FEM_PREDICTOR_CUBIC (the fem_opts_default choice: 4 x_n -
                             6 x_{n-1} + 4 x_{n-2} - x_{n-3}; one more state-sized history
                             vector), FEM_PREDICTOR_QUADRATIC (3 x_n - 3 x_{n-1} + x_{n-2}),
-                            FEM_PREDICTOR_LINEAR (0: 2 x_n - x_{n-1}) or
-                            FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
+                            FEM_PREDICTOR_QUARTIC (5 x_n - 10 x_{n-1} + 10 x_{n-2} -
+                            5 x_{n-3} + x_{n-4}), FEM_PREDICTOR_LINEAR (0: 2 x_n -
+                            x_{n-1}) or FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
                             Gray-Scott take any order; the wave equation is linear at most,
                             Newmark uses its acceleration. */
 } fem_solver_opts;
 
-enum { FEM_PREDICTOR_LINEAR = 0, FEM_PREDICTOR_PREVIOUS = 1, FEM_PREDICTOR_QUADRATIC = 2, FEM_PREDICTOR_CUBIC = 3 };
+enum {
+  FEM_PREDICTOR_LINEAR = 0,
+  FEM_PREDICTOR_PREVIOUS = 1,
+  FEM_PREDICTOR_QUADRATIC = 2,
+  FEM_PREDICTOR_CUBIC = 3,
+  FEM_PREDICTOR_QUARTIC = 4
+};
 
 /* How the workgroups of one team exchange vectors between barriers:
  * COHERENT: device-coherent loads/stores, fence-free barriers (latency-bound

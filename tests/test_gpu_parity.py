@@ -547,7 +547,8 @@ def test_stream_engine_multi_rhs(gpu, eq, krylov, k, prec):
     b.close()
 
 
-@pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC, _lib.FEM_PREDICTOR_CUBIC])
+@pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC, _lib.FEM_PREDICTOR_CUBIC,
+                                  _lib.FEM_PREDICTOR_QUARTIC])
 @pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream"])
 def test_predictors(gpu, case, pred):
     """The linear (2 x_n - x_{n-1}) and quadratic (3 x_n - 3 x_{n-1} +
