@@ -277,6 +277,9 @@ void fill_dev(fem_batch* b, InstHost& h) {
     else if (o.predictor == FEM_PREDICTOR_QUADRATIC) I.extrap = first_order ? 2 : 1;
     else if (o.predictor == FEM_PREDICTOR_CUBIC) I.extrap = !first_order ? 1 : h.d_h3 ? 3 : 2;
     else if (o.predictor == FEM_PREDICTOR_QUARTIC) I.extrap = !first_order ? 1 : h.d_h4 ? 4 : h.d_h3 ? 3 : 2;
+  } else if (o.warm_start && h.eq_override < 0 && h.eq == FEM_WAVE_NEWMARK && o.predictor != FEM_PREDICTOR_PREVIOUS) {
+    // Newmark extrapolates the acceleration (history in h3, h4): linear or quadratic
+    I.extrap = (o.predictor != FEM_PREDICTOR_LINEAR && h.d_h4) ? 2 : h.d_h3 ? 1 : 0;
   }
   I.hist = std::min(h.hist, I.extrap);
 }
@@ -589,15 +592,20 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
       b->plan_ready = false;
     }
     // the cubic predictor's third history vector, for the equations that use it
-    const bool hi_order = b->opts.predictor == FEM_PREDICTOR_CUBIC || b->opts.predictor == FEM_PREDICTOR_QUARTIC;
-    if (hi_order && (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)) {
-      for (double** hp : {&h.d_h3, &h.d_h4}) {
-        if (*hp || (hp == &h.d_h4 && b->opts.predictor != FEM_PREDICTOR_QUARTIC)) continue;
-        int rc = 0;
-        *hp = dalloc<double>((size_t)h.ctx->n * k, &rc);
-        if (rc) return rc;
-        FEM_HIP(hipMemsetAsync(*hp, 0, (size_t)h.ctx->n * k * sizeof(double), b->stream));
-      }
+    // history vectors beyond `a` and `v` the predictor needs (h3, h4)
+    const int pr = b->opts.predictor;
+    int nh = 0;
+    if (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)
+      nh = pr == FEM_PREDICTOR_CUBIC ? 1 : pr == FEM_PREDICTOR_QUARTIC ? 2 : 0;
+    else if (eq == FEM_WAVE_NEWMARK)
+      nh = pr == FEM_PREDICTOR_PREVIOUS ? 0 : pr == FEM_PREDICTOR_LINEAR ? 1 : 2;
+    for (int q = 0; q < nh; q++) {
+      double** hp = q == 0 ? &h.d_h3 : &h.d_h4;
+      if (*hp) continue;
+      int rc = 0;
+      *hp = dalloc<double>((size_t)h.ctx->n * k, &rc);
+      if (rc) return rc;
+      FEM_HIP(hipMemsetAsync(*hp, 0, (size_t)h.ctx->n * k * sizeof(double), b->stream));
     }
     if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;

@@ -427,16 +427,24 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
         x0 = warm ? (ex ? 2.0 * vo - ld<C>(I.a + o) : vo) : 0.0;
       } break;
       case EQ_NEWMARK: {  // b = -c^2 K (u + dt v + (1/2 - beta) dt^2 a)
+        // guess: the acceleration extrapolated (a_{n-1} in h3, a_{n-2} in h4)
         const double h = (0.5 - I.p2) * I.dt * I.dt;
         const D2 g = row_gather(lo, hi, D2{0.0, 0.0}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + jr;
           const double ac = ld<C>(I.a + oc);
+          const double xc = lv == 2 ? 3.0 * (ac - ld<C>(I.h3 + oc)) + ld<C>(I.h4 + oc)
+                            : ex    ? 2.0 * ac - ld<C>(I.h3 + oc)
+                                    : ac;
           return D2{K64[s] * (ld<C>(I.u + oc) + I.dt * ld<C>(I.v + oc) + h * ac),
-                    warm ? S64[(size_t)s * kv + vj] * ac : 0.0};
+                    warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
         });
         b = -I.p1 * I.p1 * g.a;
         sx = g.b;
-        x0 = warm ? ld<C>(I.a + o) : 0.0;
+        const double ao = ld<C>(I.a + o);
+        x0 = !warm     ? 0.0
+             : lv == 2 ? 3.0 * (ao - ld<C>(I.h3 + o)) + ld<C>(I.h4 + o)
+             : ex      ? 2.0 * ao - ld<C>(I.h3 + o)
+                       : ao;
       } break;
       default: {  // EQ_SOLVE: b supplied, x holds the guess
         b = gp(I.b)[o];
@@ -995,6 +1003,8 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
         const double vn = vo + dt * ((1.0 - ga) * ao + ga * xf);
         st<C>(I.u + o, un);
         st<C>(I.v + o, vn);
+        if (I.extrap >= 2) st<C>(I.h4 + o, ld<C>(I.h3 + o));  // acceleration history for the predictor
+        if (I.extrap >= 1) st<C>(I.h3 + o, ao);
         st<C>(I.a + o, xf);
         c1 = un;
         c2 = vn;

@@ -210,11 +210,18 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
           const DV ac = vld<V>(I.a + oc);
-          return P2<DV>{K64[s] * (vld<V>(I.u + oc) + I.dt * vld<V>(I.v + oc) + h * ac), warm ? S64[s] * ac : zd};
+          const DV xc = lv == 2 ? 3.0 * (ac - vld<V>(I.h3 + oc)) + vld<V>(I.h4 + oc)
+                        : ex    ? 2.0 * ac - vld<V>(I.h3 + oc)
+                                : ac;
+          return P2<DV>{K64[s] * (vld<V>(I.u + oc) + I.dt * vld<V>(I.v + oc) + h * ac), warm ? S64[s] * xc : zd};
         });
         b = -I.p1 * I.p1 * g.a;
         sx = g.b;
-        x0 = warm ? vld<V>(I.a + o) : zd;
+        const DV ao = vld<V>(I.a + o);
+        x0 = !warm     ? zd
+             : lv == 2 ? 3.0 * (ao - vld<V>(I.h3 + o)) + vld<V>(I.h4 + o)
+             : ex      ? 2.0 * ao - vld<V>(I.h3 + o)
+                       : ao;
       } break;
       default: {  // EQ_SOLVE: b supplied, x holds the guess
         b = vld<V>(I.b + o);

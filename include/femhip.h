@@ -132,8 +132,8 @@ typedef struct fem_solver_opts {
                             FEM_PREDICTOR_QUARTIC (5 x_n - 10 x_{n-1} + 10 x_{n-2} -
                             5 x_{n-3} + x_{n-4}), FEM_PREDICTOR_LINEAR (0: 2 x_n -
                             x_{n-1}) or FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
-                            Gray-Scott take any order; the wave equation is linear at most,
-                            Newmark uses its acceleration. */
+                            Gray-Scott take any order; the wave equation is linear at most;
+                            Newmark extrapolates its acceleration, quadratic at most. */
 } fem_solver_opts;
 
 enum {

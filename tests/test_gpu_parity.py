@@ -549,7 +549,8 @@ def test_stream_engine_multi_rhs(gpu, eq, krylov, k, prec):
 
 @pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC, _lib.FEM_PREDICTOR_CUBIC,
                                   _lib.FEM_PREDICTOR_QUARTIC])
-@pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream"])
+@pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream",
+                                  "newmark_persistent", "newmark_stream"])
 def test_predictors(gpu, case, pred):
     """The linear (2 x_n - x_{n-1}) and quadratic (3 x_n - 3 x_{n-1} +
     x_{n-2}, x_{n-2} kept in the v field) initial guesses change only where
@@ -574,6 +575,24 @@ def test_predictors(gpu, case, pred):
             assert sol.advance_time(n) == _lib.FEM_OK
             assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL
             assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL
+        return
+    if case.startswith("newmark"):  # the acceleration is extrapolated (linear or quadratic)
+        m = meshes.jittered_square(36, seed=5)
+        s = oracle.System(m, bc=0)
+        u = s.gather(gauss(m, (0.5, 0.45), 0.02))
+        a = s.newmark_init(1.0, u)
+        v = np.zeros(s.n)
+        prm = [1e-3, 1.0, 0.25, 0.5]
+        sol = make_solver(FEMContext(m, 0), Equation.Wave_Newmark, NewmarkParameters(*prm), predictor=Q, engine=eng)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u)
+        sol.set_state_unknowns(_lib.FEM_FIELD_A, a)
+        for n in (1, 2, 6):
+            for _ in range(n):
+                _, u, v, a = s.step(oracle.NEWMARK, prm, u, v, a, tol=1e-14)
+            assert sol.advance_time(n) == _lib.FEM_OK
+            assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < RTOL, n
+            assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < RTOL, n
+            assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_A), a) < RTOL, n
         return
     if case == "advdiff16_stream":
         k, vel = 16, (0.5, 0.3, 0.0)
