@@ -78,14 +78,6 @@ constexpr bool staged_phase() {
   return PH == PH_PCG_INIT || PH == PH_PCG_ITER || PH == PH_CG_A || PH == PH_BI_P || PH == PH_BI_S;
 }
 
-// Launched block -> position in the block list, so that each XCD (workgroups
-// are dealt to the 8 XCDs round-robin) walks one contiguous range of row
-// blocks: the halo rows of a block are then mostly rows its own XCD's L2
-// already holds.  A bijection on [0, n).
-__device__ inline int xcd_chunk(int b, int n) {
-  const int x = b & 7, q = n >> 3, r = n & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
 
 // One column per lane (the persistent kernel's phase bodies).
 template <int K, typename TI, int PH>
