@@ -688,3 +688,66 @@ def test_gl_compat_engine_vs_oracle(gpu, eq):
         assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) < 1e-4
     else:
         assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) < 1e-4
+
+
+def fan_mesh(m):
+    """A disc fanned from its centre: vertex 0 is adjacent to all m rim
+    vertices, so matrix row 0 holds m + 1 nonzeros."""
+    th = np.linspace(0.0, 2.0 * np.pi, m, endpoint=False)
+    xyz = np.zeros((m + 1, 3), dtype=np.float32)
+    xyz[1:, 0], xyz[1:, 1] = np.cos(th), np.sin(th)
+    tri = np.array([(0, 1 + i, 1 + (i + 1) % m) for i in range(m)], dtype=np.uint32)
+    return {"xyz": xyz, "tri": tri, "on_boundary": meshes.boundary_from_edges(tri, m + 1).astype(np.uint8)}
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_stream_engine_row_longer_than_block_cap(gpu, k):
+    """A row of 2201 nonzeros exceeds the 2048-nonzero row-block cap: it gets
+    a block of its own and a stage sized to it; results match the oracle."""
+    print("case fan 2200", k, flush=True)
+    m = fan_mesh(2200)
+    s = oracle.System(m, bc=1)
+    assert int(np.diff(s.row_ptr).max()) == 2201
+    rng = np.random.default_rng(13)
+    u0 = rng.normal(size=(s.n, k))
+    b = Batch([FEMContext(m, 1)], SolverOptions(nrhs=k, engine=_lib.FEM_ENGINE_STREAM))
+    b.set_equation(0, HeatParameters(1e-2, 0.3))
+    b.set_state_unknowns(0, _lib.FEM_FIELD_U, u0)
+    assert b.advance(3) == _lib.FEM_OK
+    got = b.get_state_unknowns(0).reshape(s.n, k)
+    for j in range(k):
+        _, u, _, _ = s.run(oracle.HEAT, [1e-2, 0.3], 3, u0[:, j].copy(), tol=1e-14)
+        assert rel(got[:, j], u) < RTOL, j
+    b.close()
+
+
+def test_stream_engine_refuses_row_beyond_lds(gpu):
+    """A 5401-nonzero row in FP64 needs a 65 KB stage, beyond a workgroup's
+    64 KiB of LDS: the streaming engine refuses with a clear error, and the
+    persistent engine solves the same system."""
+    print("case fan 5400", flush=True)
+    m = fan_mesh(5400)
+    s = oracle.System(m, bc=1)
+    u0 = np.random.default_rng(14).normal(size=s.n)
+    ctx = FEMContext(m, 1)
+    st = make_solver(ctx, Equation.Heat, HeatParameters(1e-2, 0.3), engine=_lib.FEM_ENGINE_STREAM,
+                     precision=_lib.FEM_PREC_FP64)
+    st.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    with pytest.raises(_lib.FemError, match="LDS"):
+        st.advance_time(1)
+    pe = make_solver(ctx, Equation.Heat, HeatParameters(1e-2, 0.3), precision=_lib.FEM_PREC_FP64)
+    pe.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    assert pe.advance_time(1) == _lib.FEM_OK
+    _, u, _, _ = s.run(oracle.HEAT, [1e-2, 0.3], 1, u0.copy(), tol=1e-14)
+    assert rel(pe.get_state_unknowns(), u) < RTOL
+
+
+def test_graph_capture_needs_persistent_engine(gpu):
+    m = meshes.jittered_square(8, seed=1)
+    b = Batch([FEMContext(m, 0)], SolverOptions(engine=_lib.FEM_ENGINE_STREAM))
+    b.set_equation(0, HeatParameters(1e-3, 1.0))
+    with pytest.raises(_lib.FemError, match="persistent"):
+        b.graph_capture(2)
+    with pytest.raises(_lib.FemError):
+        b.graph_launch()
+    b.close()
