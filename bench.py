@@ -621,8 +621,8 @@ def main():
         print(json.dumps({"calib_copy_bytes_read": x.numel() * 4, "calib_copy_bytes_written": y.numel() * 4}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only (--gpus 1): sharding across GPUs is designed, not implemented",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "spmv":
         import torch  # noqa: F401  (before the library, as above)
