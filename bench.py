@@ -430,6 +430,7 @@ def run_batched_c4(args):
 
     def make(**kw):
         kw.setdefault("predictor", PREDICTORS[args.predictor])
+        kw.setdefault("inner_tol", args.inner_tol or 1e-4)
         b = Batch(ctxs, SolverOptions(nrhs=16, **kw))
         for i in range(len(ctxs)):
             b.set_equation(i, prms[i])
@@ -604,7 +605,8 @@ def main():
                     help="grayscott, batched: initial guess of each solve")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="FP32 sweep reduction (grayscott: 1e-5, batched: the library's 1e-4)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -650,7 +652,7 @@ def main():
     ctx.parameters[Equation.Reaction_Diffusion] = prm
 
     def solver(**kw):
-        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol, team_size=args.team_size,
+        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol or 1e-5, team_size=args.team_size,
                           predictor=PREDICTORS[args.predictor])
         for k_, v_ in kw.items():
             setattr(o, k_, v_)
