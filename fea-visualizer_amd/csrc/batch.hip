@@ -799,13 +799,13 @@ int batch_prepare(fem_batch* b, bool need_plan) {
       set_error("advance: set_equation was not called for every instance");
       return FEM_ERR_STATE;
     }
-    // persistent plans of first-order equations fuse each update with the
-    // next right-hand side: two more state-sized vectors (Inst::xnext, hpre)
-    if (need_plan && !h.d_xnext &&
-        (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION)) {
+    // first-order equations form the next initial guess with each update
+    // (Inst::xnext); persistent plans also fuse the update with the next
+    // right-hand side, which needs the history term (Inst::hpre)
+    if (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION) {
       int rc = 0;
-      h.d_xnext = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
-      if (!rc) h.d_hpre = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (!h.d_xnext) h.d_xnext = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (!rc && need_plan && !h.d_hpre) h.d_hpre = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (rc) return rc;
     }
     if (h.dirty || h.formed_version != h.ctx->version) {

@@ -959,8 +959,12 @@ __device__ void phase_bi_s_c(const Inst& I, int row_lo, int row_hi, const TI* r,
 
 // State update and the instability rule (CPUSolver.cpp:41-48, widened to
 // flag NaN/Inf separately).
+// With lvn >= 0 (heat, advection-diffusion, Gray-Scott), also the next
+// step's initial guess of predictor order lvn into xnext, from this row's own
+// history: the next right-hand side then gathers one vector for S x0.
 template <int K, typename TI, int C>
-__device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags, int step) {
+__device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags, int step,
+                             int lvn = -1) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   TI* xi = (TI*)I.xi;
@@ -980,10 +984,19 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_ADVDIFF:
       case EQ_GS:
         if (I.extrap) {  // history for the predictor: x_{n-4} <- x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n
-          if (I.extrap >= 4) st<C>(I.h4 + o, ld<C>(I.h3 + o));
-          if (I.extrap >= 3) st<C>(I.h3 + o, ld<C>(I.v + o));
-          if (I.extrap >= 2) st<C>(I.v + o, ld<C>(I.a + o));
-          st<C>(I.a + o, ld<C>(I.u + o));
+          const double uo = ld<C>(I.u + o);
+          const double ao = I.extrap >= 2 ? ld<C>(I.a + o) : 0.0;
+          const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
+          const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
+          if (I.extrap >= 4) st<C>(I.h4 + o, ho);
+          if (I.extrap >= 3) st<C>(I.h3 + o, vo);
+          if (I.extrap >= 2) st<C>(I.v + o, ao);
+          st<C>(I.a + o, uo);
+          if (lvn >= 0 && I.xnext) {
+            double w0, w1, w2, w3, w4;
+            pred_weights(lvn, w0, w1, w2, w3, w4);
+            gp(I.xnext)[o] = w0 * xf + w1 * uo + w2 * ao + w3 * vo + w4 * ho;
+          }
         }
         st<C>(I.u + o, xf);
         break;

@@ -58,6 +58,7 @@ struct SArgs {
   int first;            // first iteration of the sweep
   int step;             // step index inside the launch sequence (trajectory records)
   int pass;             // FP64 residual: 0 normal, 1 redo with the correction dropped
+  int more;             // update: another step follows in this launch (it reads xnext)
   double* partials;     // [nblocks][3 K]
   const int* act_blk;   // launched block -> row block (null: every row block, in order)
   const int* act_inst;  // reduction block -> instance (null: every instance)
@@ -203,7 +204,8 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   constexpr int V = vec_cols<K, TI>();
   if constexpr (PH == PH_UPD) {
     __syncthreads();
-    phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step);
+    const int lvn = (a.more && I.warm) ? min(I.extrap, I.hist + a.step + 1) : -1;
+    phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
   } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
     // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
     extern __shared__ int dyn[];
@@ -836,6 +838,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       STREAM_PH(PH_RES, cv);
       a.pass = 0;
     }
+    a.more = step + 1 < nsteps;
     STREAM_PH(PH_UPD, full);
     hipLaunchKernelGGL(k_step_end, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts, p->ninst,
                        p->k);

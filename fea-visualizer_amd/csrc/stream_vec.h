@@ -150,6 +150,9 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
   const bool warm = I.warm != 0;
   const int lv = warm ? min(I.extrap, I.hist + step) : 0;  // predictor order this step
   const bool ex = lv > 0;
+  // after the first step of a launch the update wrote this step's guess into
+  // xnext (one-column-per-lane update phase, stream.hip)
+  const bool pre = ex && step > 0 && I.xnext && I.eq != EQ_GS;
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -164,6 +167,16 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS: {  // b = (M/dt) u                                  CPUSolver.cpp:78,96
+        if (pre) {  // the previous update formed the guess (xnext): gather it alone
+          const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
+            const size_t oc = (size_t)ci[s] * K + j0;
+            return P2<DV>{M64[s] * vld<V>(I.u + oc), S64[s] * vld<V>(I.xnext + oc)};
+          });
+          b = g.a * inv_dt;
+          sx = g.b;
+          x0 = vld<V>(I.xnext + o);
+          break;
+        }
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
           const DV uc = vld<V>(I.u + oc);
