@@ -169,7 +169,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv_stream(int n, long long nnz, co
     return acc;
   };
   const vec acc = staged ? walk(s_ci, s_val, base) : walk(ci, val, 0);
-  *(vec*)(y + (size_t)row * K + c) = acc;
+  if constexpr (NT) __builtin_nontemporal_store(acc, (vec*)(y + (size_t)row * K + c));  // y is not re-read here
+  else *(vec*)(y + (size_t)row * K + c) = acc;
 }
 
 template <typename T, int K>
