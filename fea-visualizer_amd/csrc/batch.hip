@@ -98,7 +98,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 
 void free_inst(InstHost& h) {
   void* ptrs[] = {h.d_S64, h.d_Si,    h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v,   h.d_a,
-                  h.d_h3,  h.d_h4,    h.d_xnext,  h.d_hpre,  h.d_x,   h.d_b, h.d_vec,
+                  h.d_h3,  h.d_h4,    h.d_xnext,  h.d_hpre,  h.d_hvec, h.d_x, h.d_b, h.d_vec,
                   h.d_status, h.d_counters, h.d_last_res};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -257,6 +257,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.h4 = h.d_h4;
   I.xnext = h.d_xnext;
   I.hpre = h.d_hpre;
+  I.hvec = h.d_hvec;
   I.x = h.d_x;
   I.b = h.d_b;
   char* base = (char*)h.d_vec;
@@ -801,11 +802,13 @@ int batch_prepare(fem_batch* b, bool need_plan) {
     }
     // first-order equations form the next initial guess with each update
     // (Inst::xnext); persistent plans also fuse the update with the next
-    // right-hand side, which needs the history term (Inst::hpre)
+    // right-hand side, which needs the history term (Inst::hpre, formed from
+    // Inst::hvec)
     if (h.eq == FEM_HEAT || h.eq == FEM_ADVECTION_DIFFUSION || h.eq == FEM_REACTION_DIFFUSION) {
       int rc = 0;
       if (!h.d_xnext) h.d_xnext = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (!rc && need_plan && !h.d_hpre) h.d_hpre = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (!rc && need_plan && !h.d_hvec) h.d_hvec = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (rc) return rc;
     }
     if (h.dirty || h.formed_version != h.ctx->version) {

@@ -100,6 +100,8 @@ struct Inst {
   double* h4;            // [n][k] x_{n-4} (quartic predictor: 5 x_n - 10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4})
   double* xnext;         // [n][k] next initial guess formed with the update (persistent kernel)
   double* hpre;          // [n][k] history part of the next initial residual
+  double* hvec;          // [n][k] history part of the next initial guess, formed with the update:
+                         //   phase_res gathers it instead of the history levels
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

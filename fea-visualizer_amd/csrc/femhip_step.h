@@ -58,6 +58,7 @@ struct InstHost {
   double* d_h4 = nullptr;
   double* d_xnext = nullptr; // Inst::xnext, Inst::hpre
   double* d_hpre = nullptr;
+  double* d_hvec = nullptr;  // Inst::hvec
   double* d_S64 = nullptr;
   void* d_Si = nullptr;
   double* d_dinv64 = nullptr;

@@ -487,7 +487,7 @@ __device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, 
 template <int K, typename TI, int C>
 __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, int row_lo, int row_hi, TI* r,
                                                                 const Shared& sh, double (&acc)[2], int lvn = 0,
-                                                                double* hpre = nullptr) {
+                                                                double* hpre = nullptr, bool hready = false) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -504,8 +504,17 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
     const double bo = gp(I.b)[o];
+    const TI dj = dinv[(size_t)row * kv + vj];  // loaded before the stores below (one completion counter)
     double sx;
-    if (want_h) {
+    if (want_h && hready) {  // the update formed the history combination in hvec
+      const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
+        const size_t oc = (size_t)ci[s] * K + jr;
+        const double sv = S64[(size_t)s * kv + vj];
+        return D2{sv * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc))), sv * ld<C>(I.hvec + oc)};
+      });
+      sx = g.a;
+      gp(hpre)[o] = g.b;
+    } else if (want_h) {
       const D2 g = row_gather<kUWide>(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
         const size_t oc = (size_t)ci[s] * K + jr;
         const double sv = S64[(size_t)s * kv + vj];
@@ -526,7 +535,7 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
     st<C>(r + o, rr);
     if (drop) st<C>(xi + o, (TI)0);
     acc[0] += r64 * r64;
-    acc[1] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
+    acc[1] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dj * rr);
   }
 }
 
@@ -545,6 +554,7 @@ __device__ void phase_pcg_init(const Inst& I, int row_lo, int row_hi, const TI* 
     const size_t o = (size_t)row * K + jr;
     const double xo = ld<C>(I.x + o), xio = (double)ld<C>(xi + o);
     const TI ro = ld<C>(rc + o);
+    const TI uo = dinv[(size_t)row * kv + vj] * ro;  // loaded before the stores below
     const TI wa = row_gather(rp[row], rp[row + 1], (TI)0, [&](int s) {
       const int c = ci[s];
       return S[(size_t)s * kv + vj] * (dinv[(size_t)c * kv + vj] * ld<C>(rc + (size_t)c * K + jr));
@@ -552,7 +562,6 @@ __device__ void phase_pcg_init(const Inst& I, int row_lo, int row_hi, const TI* 
     st<C>(I.x + o, xo + xio);
     st<C>(xi + o, (TI)0);
     st<C>(wc + o, wa);
-    const TI uo = dinv[(size_t)row * kv + vj] * ro;
     acc[0] += (double)ro * (double)uo;
     acc[1] += (double)uo * (double)wa;
   }
@@ -819,6 +828,7 @@ __device__ void phase_pcg_init_c(const Inst& I, int row_lo, int row_hi, const TI
     const size_t o = (size_t)row * K + jr;
     const double xo = ld<C>(I.x + o), xio = (double)ld<C>(xi + o);
     const TI ro = ld<C>(rc + o);
+    const TI uo = dinv[(size_t)row * kv + vj] * ro;  // loaded before the stores below
     TI g[kCacheE];
 #pragma unroll
     for (int e = 0; e < kCacheE; e++) g[e] = ld<C>(rc + (size_t)RC.col[q][e][threadIdx.x] * K + jr);
@@ -828,7 +838,6 @@ __device__ void phase_pcg_init_c(const Inst& I, int row_lo, int row_hi, const TI
     st<C>(I.x + o, xo + xio);
     st<C>(xi + o, (TI)0);
     st<C>(wc + o, wa);
-    const TI uo = dinv[(size_t)row * kv + vj] * ro;
     acc[0] += (double)ro * (double)uo;
     acc[1] += (double)uo * (double)wa;
   }
@@ -972,9 +981,16 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
   double* traj = nullptr;  // trajectory record of this step, if any
   if (I.traj && (step + 1) % I.traj_every == 0)
     traj = I.traj + (size_t)((step + 1) / I.traj_every - 1) * I.n * K;
+  const bool first_order = I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF || I.eq == EQ_GS;
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
+    // every load of the row first: loads and stores share one completion
+    // counter, so a load issued after a store also waits for the store
     double xf = ld<C>(I.x + o) + (double)ld<C>(xi + o);
+    const double uo = ld<C>(I.u + o);
+    const double ao = (first_order ? I.extrap >= 2 : I.eq == EQ_NEWMARK) ? ld<C>(I.a + o) : 0.0;
+    const double vo = (first_order ? I.extrap >= 3 : (I.eq == EQ_WAVE || I.eq == EQ_NEWMARK)) ? ld<C>(I.v + o) : 0.0;
+    const double ho = (first_order ? I.extrap >= 4 : (I.eq == EQ_NEWMARK && I.extrap >= 2)) ? ld<C>(I.h3 + o) : 0.0;
     if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
     st<C>(I.x + o, xf);
     st<C>(xi + o, (TI)0);
@@ -984,10 +1000,6 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
       case EQ_ADVDIFF:
       case EQ_GS:
         if (I.extrap) {  // history for the predictor: x_{n-4} <- x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n
-          const double uo = ld<C>(I.u + o);
-          const double ao = I.extrap >= 2 ? ld<C>(I.a + o) : 0.0;
-          const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
-          const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
           if (I.extrap >= 4) st<C>(I.h4 + o, ho);
           if (I.extrap >= 3) st<C>(I.h3 + o, vo);
           if (I.extrap >= 2) st<C>(I.v + o, ao);
@@ -1001,22 +1013,21 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
         st<C>(I.u + o, xf);
         break;
       case EQ_WAVE: {
-        if (I.extrap) st<C>(I.a + o, ld<C>(I.v + o));
+        if (I.extrap) st<C>(I.a + o, vo);
         st<C>(I.v + o, xf);
-        const double un = ld<C>(I.u + o) + xf * I.dt;  // CPUSolver.cpp:118
+        const double un = uo + xf * I.dt;  // CPUSolver.cpp:118
         st<C>(I.u + o, un);
         c2 = un;
         up = un;
       } break;
       case EQ_NEWMARK: {
         const double dt = I.dt, be = I.p2, ga = I.p3;
-        const double ao = ld<C>(I.a + o), vo = ld<C>(I.v + o);
-        const double ut = ld<C>(I.u + o) + dt * vo + (0.5 - be) * dt * dt * ao;
+        const double ut = uo + dt * vo + (0.5 - be) * dt * dt * ao;
         const double un = ut + be * dt * dt * xf;
         const double vn = vo + dt * ((1.0 - ga) * ao + ga * xf);
         st<C>(I.u + o, un);
         st<C>(I.v + o, vn);
-        if (I.extrap >= 2) st<C>(I.h4 + o, ld<C>(I.h3 + o));  // acceleration history for the predictor
+        if (I.extrap >= 2) st<C>(I.h4 + o, ho);  // acceleration history for the predictor
         if (I.extrap >= 1) st<C>(I.h3 + o, ao);
         st<C>(I.a + o, xf);
         c1 = un;
@@ -1043,7 +1054,7 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
 // phase_hpre of this step); lvn is the next step's predictor order.
 template <int K, typename TI, int C>
 __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, const Shared& sh, int* flags, int step,
-                                 int lvn, double (&acc)[3]) {
+                                 int lvn, int lvh, double (&acc)[3]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -1053,6 +1064,8 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
   const bool warm = I.warm != 0;
   double w0, w1, w2, w3, w4;
   pred_weights(lvn, w0, w1, w2, w3, w4);
+  double h0, h1, h2, h3, h4;  // lvh: the step after this one (its history combination goes to hvec)
+  pred_weights(lvh, h0, h1, h2, h3, h4);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -1067,22 +1080,15 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
   };
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
+    // every load of the row first: loads and stores share one completion
+    // counter, so a load issued after a store also waits for the store
     const double xn = xn_at(o, jr);
-    // ---- update: state and history x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n (own row)
     const double uo = ld<C>(I.u + o);
     const double ao = I.extrap >= 2 ? ld<C>(I.a + o) : 0.0;
     const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
     const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
-    if (I.extrap >= 4) st<C>(I.h4 + o, ho);
-    if (I.extrap >= 3) st<C>(I.h3 + o, vo);
-    if (I.extrap >= 2) st<C>(I.v + o, ao);
-    if (I.extrap >= 1) st<C>(I.a + o, uo);
-    st<C>(I.u + o, xn);
-    if (traj) gp(traj)[o] = xn;
-    if (xn > 1e4) f |= ST_UNSTABLE;
-    if (!isfinite(xn)) f |= ST_NONFINITE;
     // ---- next right-hand side: b = (M/dt) x_n (+ Gray-Scott reaction), r = b - S x0
-    const D2 g = row_gather<kUWide>(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
+    const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
       const double xc = xn_at((size_t)ci[s] * K + jr, jr);
       return D2{M64[s] * xc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
     });
@@ -1094,16 +1100,29 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
       const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
       b += w * fr;
     }
+    const double hp = (warm && lvn > 0) ? gp(I.hpre)[o] : 0.0;
+    const TI dj = dinv[(size_t)row * kv + vj];
     const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo + w4 * ho : 0.0;
-    const double sx = warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0;
+    const double sx = warm ? w0 * g.b + hp : 0.0;
     const double r64 = b - sx;
+    const TI rr = to_ti<TI>(r64);
+    // ---- update: state and history x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n (own row)
+    if (I.extrap >= 4) st<C>(I.h4 + o, ho);
+    if (I.extrap >= 3) st<C>(I.h3 + o, vo);
+    if (I.extrap >= 2) st<C>(I.v + o, ao);
+    if (I.extrap >= 1) st<C>(I.a + o, uo);
+    st<C>(I.u + o, xn);
+    // the shifted history is x_n, x_{n-1}, x_{n-2}, x_{n-3}
+    if (lvh > 0) st<C>(I.hvec + o, h1 * xn + h2 * uo + h3 * ao + h4 * vo);
+    if (traj) gp(traj)[o] = xn;
+    if (xn > 1e4) f |= ST_UNSTABLE;
+    if (!isfinite(xn)) f |= ST_NONFINITE;
     gp(I.b)[o] = b;
     gp(I.xnext)[o] = x0;
-    const TI rr = to_ti<TI>(r64);
     st<C>(r + o, rr);
     acc[0] += b * b;
     acc[1] += r64 * r64;
-    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dinv[(size_t)row * kv + vj] * rr);
+    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dj * rr);
   }
   if (f) atomicOr(flags, f);
 }
@@ -1124,7 +1143,7 @@ __device__ void phase_install(const Inst& I, int row_lo, int row_hi) {
 // The history part of the next step's initial residual (as phase_res does)
 // for a step that ran no refinement sweep.
 template <int K, typename TI, int C>
-__device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn) {
+__device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn, bool hready) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -1135,6 +1154,12 @@ __device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn) {
   const double GA* S64 = gp(I.S64);
   for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
     const size_t o = (size_t)row * K + jr;
+    if (hready) {
+      gp(I.hpre)[o] = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
+        return S64[(size_t)s * kv + vj] * ld<C>(I.hvec + (size_t)ci[s] * K + jr);
+      });
+      continue;
+    }
     gp(I.hpre)[o] = row_gather<kUWide>(rp[row], rp[row + 1], 0.0, [&](int s) {
       const size_t oc = (size_t)ci[s] * K + jr;
       const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
@@ -1372,7 +1397,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       double acc[3] = {0.0, 0.0, 0.0};
       if (have_rhs)
         phase_update_rhs<K, TI, C>(I, row_lo, row_hi, r0, sh, I.status, step - 1,
-                                   I.warm ? min(I.extrap, I.hist + step) : 0, acc);
+                                   I.warm ? min(I.extrap, I.hist + step) : 0, I.hvec ? lvn : 0, acc);
       else
         phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
@@ -1389,6 +1414,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     __syncthreads();
     // own rows only: the phases up to the next barrier read no other row's x
     if (have_rhs) phase_install<K, TI, C>(I, row_lo, row_hi);
+    const bool hready = have_rhs && I.hvec;  // the update above formed hvec
     int sweeps = 0, its = 0;
     while (true) {
       if (threadIdx.x == 0) {
@@ -1416,7 +1442,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       // correction and takes the residual of x alone.
       for (int pass = 0; pass < 2; pass++) {
         double acc[2] = {0.0, 0.0};
-        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc, lvn, fuse_next ? I.hpre : nullptr);
+        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc, lvn, fuse_next ? I.hpre : nullptr, hready);
         if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
         if (threadIdx.x == 0) {
           int redo = 0;
@@ -1455,7 +1481,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       // ---- U deferred: it runs with the next step's right-hand side.  With
       // no FP64 residual this step, form the history term here and share the
       // installed x before neighbours gather it.
-      if (sweeps == 0 && lvn > 0) phase_hpre<K, TI, C>(I, row_lo, row_hi, lvn);
+      if (sweeps == 0 && lvn > 0) phase_hpre<K, TI, C>(I, row_lo, row_hi, lvn, hready);
       barrier = sweeps == 0 && have_rhs;
       have_rhs = true;
     } else {
