@@ -479,11 +479,39 @@ __device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, 
   w4 = lv == 4 ? 1.0 : 0.0;
 }
 
-// FP64 refinement residual r = b - S64 (x + xi).  With hpre, also the
-// history part S64 (w1 u + w2 a + w3 v + w4 h3) of the next step's initial residual
-// for predictor order lvn (the state update of this step and the right-hand
-// side of the next run as one phase, phase_update_rhs, which cannot gather
-// history its neighbours are shifting).
+// One row of the FP64 residual pass of a fused step: S c, M c and S h, with
+// c = x + xi the candidate solution (xi dropped for a column restarted from x)
+// and h the next initial guess's history part.  HM: 0 no history part,
+// 1 h gathered from hvec, 2 h formed from the history levels u, a, v, h3.
+template <int HM, int K, typename TI, int C>
+__device__ __attribute__((always_inline)) inline D3 res_gather_fused(const Inst& I, int lo, int hi, int jr, int vj,
+                                                                     bool drop, int lvn, double w1, double w2,
+                                                                     double w3, double w4) {
+  const int kv = I.kv;
+  const TI* xi = (const TI*)I.xi;
+  const int GA* __restrict__ ci = gp(I.ci);
+  const double GA* S64 = gp(I.S64);
+  const double GA* M64 = gp(I.M64);
+  return row_gather<kUWide>(lo, hi, D3{0.0, 0.0, 0.0}, [&](int s) {
+    const size_t oc = (size_t)ci[s] * K + jr;
+    const double sv = S64[(size_t)s * kv + vj];
+    const double xc = ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc));
+    double hv = 0.0;
+    if constexpr (HM == 1) hv = ld<C>(I.hvec + oc);
+    if constexpr (HM == 2)
+      hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
+           (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0) + (lvn >= 4 ? w4 * ld<C>(I.h3 + oc) : 0.0);
+    return D3{sv * xc, M64[s] * xc, sv * hv};
+  });
+}
+
+// FP64 residual of x + xi into r (inner precision); acc: <r, r>, <r, D^-1 r>.
+// hpre != nullptr: the next step's update is fused with its right-hand side
+// and takes, should this pass turn out to be the last of the step, the
+// products of the candidate solution from here instead of gathering it:
+// M (x + xi) into xnext and the S-part of the next initial residual,
+// w0 S (x + xi) + S h, into hpre (lvn: the next step's predictor order;
+// hready: the update of this step formed h in hvec).
 template <int K, typename TI, int C>
 __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, int row_lo, int row_hi, TI* r,
                                                                 const Shared& sh, double (&acc)[2], int lvn = 0,
@@ -494,6 +522,9 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
   TI* xi = (TI*)I.xi;
   // a column whose sweep produced non-finite values restarts from x alone
   const bool drop = sh.reset[jr] != 0;
+  // Eigen: rhs == 0 -> x = 0, so the next update sees a zero solution
+  const bool zero = sh.bb[jr] == 0.0;
+  const bool warm = I.warm != 0;
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
@@ -506,24 +537,15 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
     const double bo = gp(I.b)[o];
     const TI dj = dinv[(size_t)row * kv + vj];  // loaded before the stores below (one completion counter)
     double sx;
-    if (want_h && hready) {  // the update formed the history combination in hvec
-      const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
-        const size_t oc = (size_t)ci[s] * K + jr;
-        const double sv = S64[(size_t)s * kv + vj];
-        return D2{sv * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc))), sv * ld<C>(I.hvec + oc)};
-      });
+    if (hpre) {
+      const int lo = rp[row], hi = rp[row + 1];
+      const D3 g = !want_h ? res_gather_fused<0, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4)
+                   : hready ? res_gather_fused<1, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4)
+                            : res_gather_fused<2, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4);
       sx = g.a;
-      gp(hpre)[o] = g.b;
-    } else if (want_h) {
-      const D2 g = row_gather<kUWide>(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
-        const size_t oc = (size_t)ci[s] * K + jr;
-        const double sv = S64[(size_t)s * kv + vj];
-        const double hv = w1 * ld<C>(I.u + oc) + (lvn >= 2 ? w2 * ld<C>(I.a + oc) : 0.0) +
-                          (lvn >= 3 ? w3 * ld<C>(I.v + oc) : 0.0) + (lvn >= 4 ? w4 * ld<C>(I.h3 + oc) : 0.0);
-        return D2{sv * (ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc))), sv * hv};
-      });
-      sx = g.a;
-      gp(hpre)[o] = g.b;
+      const double su = zero ? 0.0 : g.a;
+      gp(hpre)[o] = warm ? w0 * su + (want_h ? g.c : 0.0) : 0.0;
+      gp(I.xnext)[o] = zero ? 0.0 : g.b;
     } else {
       sx = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
         const size_t oc = (size_t)ci[s] * K + jr;
@@ -1051,10 +1073,13 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
 // workgroup's own rows: the state and predictor history, b, r, and the next
 // initial guess in xnext, which phase_install moves into x (xi = 0) after the
 // barrier.  The history part of S x0 comes from hpre (phase_res or
-// phase_hpre of this step); lvn is the next step's predictor order.
+// phase_hpre of this step); lvn is the next step's predictor order.  pre:
+// the last residual pass of this step already formed M x_n (in xnext) and
+// all of S x0 (in hpre), so no neighbour is gathered.  lvh > 0: also the
+// history part of the initial guess after that into hvec.
 template <int K, typename TI, int C>
 __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, const Shared& sh, int* flags, int step,
-                                 int lvn, int lvh, double (&acc)[3]) {
+                                 int lvn, int lvh, bool pre, double (&acc)[3]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -1088,11 +1113,19 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
     const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
     const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
     // ---- next right-hand side: b = (M/dt) x_n (+ Gray-Scott reaction), r = b - S x0
-    const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
-      const double xc = xn_at((size_t)ci[s] * K + jr, jr);
-      return D2{M64[s] * xc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
-    });
-    double b = g.a * inv_dt;
+    double mx, sx;  // M x_n and S x0
+    if (pre) {      // both formed by the last residual pass of this step
+      mx = gp(I.xnext)[o];
+      sx = gp(I.hpre)[o];
+    } else {
+      const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
+        const double xc = xn_at((size_t)ci[s] * K + jr, jr);
+        return D2{M64[s] * xc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
+      });
+      mx = g.a;
+      sx = warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0;
+    }
+    double b = mx * inv_dt;
     if (I.eq == EQ_GS) {  // CPUSolver.cpp:134,137 with the new state
       const double uu = xn_at((size_t)row * K, 0), vv = xn_at((size_t)row * K + 1, 1);
       const double uvv = uu * (vv * vv);
@@ -1100,10 +1133,8 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
       const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
       b += w * fr;
     }
-    const double hp = (warm && lvn > 0) ? gp(I.hpre)[o] : 0.0;
     const TI dj = dinv[(size_t)row * kv + vj];
     const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo + w4 * ho : 0.0;
-    const double sx = warm ? w0 * g.b + hp : 0.0;
     const double r64 = b - sx;
     const TI rr = to_ti<TI>(r64);
     // ---- update: state and history x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n (own row)
@@ -1388,6 +1419,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
   // (each barrier has one call site: inlined copies of the wait loops spill)
   const bool fuse_eq = (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF || I.eq == EQ_GS) && I.xnext && I.hpre;
   bool have_rhs = false;  // the previous step's update is pending: it runs with this step's right-hand side
+  bool rhs_pre = false;   // ... and the previous step's last residual pass formed its products
   for (int step = 0; step < nsteps; step++) {
     const bool fuse_next = fuse_eq && step + 1 < nsteps;
     const int lvn = (fuse_next && I.warm) ? min(I.extrap, I.hist + step + 1) : 0;  // next step's predictor order
@@ -1397,7 +1429,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       double acc[3] = {0.0, 0.0, 0.0};
       if (have_rhs)
         phase_update_rhs<K, TI, C>(I, row_lo, row_hi, r0, sh, I.status, step - 1,
-                                   I.warm ? min(I.extrap, I.hist + step) : 0, I.hvec ? lvn : 0, acc);
+                                   I.warm ? min(I.extrap, I.hist + step) : 0, I.hvec ? lvn : 0, rhs_pre, acc);
       else
         phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
       if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
@@ -1484,6 +1516,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       if (sweeps == 0 && lvn > 0) phase_hpre<K, TI, C>(I, row_lo, row_hi, lvn, hready);
       barrier = sweeps == 0 && have_rhs;
       have_rhs = true;
+      rhs_pre = sweeps > 0;
     } else {
       phase_update<K, TI, C>(I, row_lo, row_hi, sh, I.status, step);
       barrier = true;  // the next step (or the next instance of this team) gathers the new state
@@ -1505,10 +1538,16 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
 // ---------------------------------------------------------------------------
 
 template <int K, typename TI, int KR, int C>
-__global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* wg_team, const Inst* insts,
-                                                const int* team_inst, int nsteps) {
+__global__ void __launch_bounds__(kBlock) k_step(const Team* __restrict__ teams, const int* __restrict__ wg_team,
+                                                const Inst* __restrict__ insts, const int* __restrict__ team_inst,
+                                                int nsteps) {
   __shared__ Shared sh;
   __shared__ RowCache<TI> RC;
+  // The instance record lives in LDS while it runs: its fields are read in
+  // every phase, and a global load of one would wait for every store the
+  // workgroup still has in flight (loads and stores share one counter).
+  __shared__ Inst sI;
+  static_assert(sizeof(Inst) % sizeof(int) == 0, "Inst copied as ints");
   const int tid_team = wg_team[blockIdx.x];
   if (tid_team < 0) return;
   const Team T = teams[tid_team];
@@ -1516,9 +1555,12 @@ __global__ void __launch_bounds__(kBlock) k_step(const Team* teams, const int* w
   const int rank = xcd_rank(l, T.size);
   unsigned e = 0;  // barrier epoch, identical across the team
   for (int qi = T.inst_begin; qi < T.inst_end; qi++) {
-    const Inst& I = insts[team_inst[qi]];
-    if (I.n <= 0) continue;
-    if (!run_instance<K, TI, KR, C>(I, T, rank, e, sh, RC, nsteps)) return;
+    __syncthreads();  // the previous instance's readers are done
+    const int* src = (const int*)(insts + team_inst[qi]);
+    for (int w = threadIdx.x; w < (int)(sizeof(Inst) / sizeof(int)); w += kBlock) ((int*)&sI)[w] = src[w];
+    __syncthreads();
+    if (sI.n <= 0) continue;
+    if (!run_instance<K, TI, KR, C>(sI, T, rank, e, sh, RC, nsteps)) return;
   }
 }
 
