@@ -77,6 +77,7 @@ struct Shared {
   double bb[kMaxK];
   int active[kMaxK];
   int uni;                 // uniform control value broadcast
+  int redo;                // refinement: repeat the residual pass
   int bkd;                 // BiCGSTAB: a scalar of this iteration broke down
   int reset[kMaxK];        // refinement: drop the correction xi of column j
 };
@@ -479,21 +480,33 @@ __device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, 
   w4 = lv == 4 ? 1.0 : 0.0;
 }
 
+// LDS row cache of a workgroup's rows (filled by fill_cache below).
+constexpr int kCacheE = 10;  // entries per cached row
+constexpr int kCacheP = 2;   // cached rows per lane
+
+template <typename TI>
+struct RowCache {
+  int col[kCacheP][kCacheE][kBlock];
+  TI val[kCacheP][kCacheE][kBlock];
+};
+
 // One row of the FP64 residual pass of a fused step: S c, M c and S h, with
 // c = x + xi the candidate solution (xi dropped for a column restarted from x)
 // and h the next initial guess's history part.  HM: 0 no history part,
 // 1 h gathered from hvec, 2 h formed from the history levels u, a, v, h3.
-template <int HM, int K, typename TI, int C>
+// cols (CC): the row's column indices in the LDS row cache, entry e at
+// cols[e * kBlock], so the gathers need not wait for the column loads.
+template <int HM, bool CC, int K, typename TI, int C>
 __device__ __attribute__((always_inline)) inline D3 res_gather_fused(const Inst& I, int lo, int hi, int jr, int vj,
                                                                      bool drop, int lvn, double w1, double w2,
-                                                                     double w3, double w4) {
+                                                                     double w3, double w4, const int* cols) {
   const int kv = I.kv;
   const TI* xi = (const TI*)I.xi;
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* S64 = gp(I.S64);
   const double GA* M64 = gp(I.M64);
   return row_gather<kUWide>(lo, hi, D3{0.0, 0.0, 0.0}, [&](int s) {
-    const size_t oc = (size_t)ci[s] * K + jr;
+    const size_t oc = (size_t)(CC ? cols[(s - lo) * kBlock] : ci[s]) * K + jr;
     const double sv = S64[(size_t)s * kv + vj];
     const double xc = ld<C>(I.x + oc) + (drop ? 0.0 : (double)ld<C>(xi + oc));
     double hv = 0.0;
@@ -511,11 +524,13 @@ __device__ __attribute__((always_inline)) inline D3 res_gather_fused(const Inst&
 // products of the candidate solution from here instead of gathering it:
 // M (x + xi) into xnext and the S-part of the next initial residual,
 // w0 S (x + xi) + S h, into hpre (lvn: the next step's predictor order;
-// hready: the update of this step formed h in hvec).
+// hready: the update of this step formed h in hvec; RC: the workgroup's
+// rows are in the LDS row cache).
 template <int K, typename TI, int C>
 __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, int row_lo, int row_hi, TI* r,
                                                                 const Shared& sh, double (&acc)[2], int lvn = 0,
-                                                                double* hpre = nullptr, bool hready = false) {
+                                                                double* hpre = nullptr, bool hready = false,
+                                                                const RowCache<TI>* RC = nullptr) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -532,16 +547,20 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
   double w0, w1, w2, w3, w4;
   pred_weights(lvn, w0, w1, w2, w3, w4);
   const bool want_h = hpre && lvn > 0;
-  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+  int q = 0;  // pass over the workgroup's rows (row cache slot)
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP, q++) {
     const size_t o = (size_t)row * K + jr;
     const double bo = gp(I.b)[o];
     const TI dj = dinv[(size_t)row * kv + vj];  // loaded before the stores below (one completion counter)
     double sx;
     if (hpre) {
       const int lo = rp[row], hi = rp[row + 1];
-      const D3 g = !want_h ? res_gather_fused<0, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4)
-                   : hready ? res_gather_fused<1, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4)
-                            : res_gather_fused<2, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4);
+      const int* cols = RC ? &RC->col[q][0][threadIdx.x] : nullptr;
+      const D3 g =
+          !want_h  ? res_gather_fused<0, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols)
+          : !hready ? res_gather_fused<2, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols)
+          : RC      ? res_gather_fused<1, true, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols)
+                    : res_gather_fused<1, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols);
       sx = g.a;
       const double su = zero ? 0.0 : g.a;
       gp(hpre)[o] = warm ? w0 * su + (want_h ? g.c : 0.0) : 0.0;
@@ -791,17 +810,9 @@ __device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, const T
 // The Krylov gathers then have one dependent memory level (the vector loads)
 // instead of three (row pointer -> column index -> vector).  Unused entries
 // point at the row itself with weight 0, so the unrolled gather is
-// branch-free and every address is valid.
+// branch-free and every address is valid (RowCache, defined with the FP64
+// residual pass, which also takes its column indices).
 // ---------------------------------------------------------------------------
-
-constexpr int kCacheE = 10;  // entries per cached row
-constexpr int kCacheP = 2;   // cached rows per lane
-
-template <typename TI>
-struct RowCache {
-  int col[kCacheP][kCacheE][kBlock];
-  TI val[kCacheP][kCacheE][kBlock];
-};
 
 // Fill the cache for this workgroup's rows; true (uniform) when every row fits.
 template <int K, typename TI>
@@ -1243,27 +1254,27 @@ __device__ bool krylov_pcg(const Inst& I, const Team& T, int rank, unsigned& e, 
     first = false;
     it_s++;
     its++;
-    if (threadIdx.x < K) {
-      const int j = threadIdx.x;
-      const double g = sh.red[j], d = sh.red[K + j], rr = sh.red[2 * K + j];
-      const double gold = sh.gamma[j], aold = sh.alpha[j];
-      const bool done = !sh.active[j] || rr < sh.ithr[j] || !(rr == rr);
-      double be = 0.0, al = 0.0;
-      if (!done && gold != 0.0 && aold != 0.0) {
-        be = g / gold;
-        const double den = d - be * g / aold;
-        al = (den != 0.0) ? g / den : 0.0;
+    // the recurrences of the K columns and the stop vote in wave 0 (K <= 64
+    // columns, one lane each), then one workgroup barrier
+    if (threadIdx.x < kWave) {
+      bool done = true;
+      if (threadIdx.x < K) {
+        const int j = threadIdx.x;
+        const double g = sh.red[j], d = sh.red[K + j], rr = sh.red[2 * K + j];
+        const double gold = sh.gamma[j], aold = sh.alpha[j];
+        done = !sh.active[j] || rr < sh.ithr[j] || !(rr == rr);
+        double be = 0.0, al = 0.0;
+        if (!done && gold != 0.0 && aold != 0.0) {
+          be = g / gold;
+          const double den = d - be * g / aold;
+          al = (den != 0.0) ? g / den : 0.0;
+        }
+        sh.beta[j] = be;
+        sh.alpha[j] = al;
+        sh.gamma[j] = g;
       }
-      sh.beta[j] = be;
-      sh.alpha[j] = al;
-      sh.gamma[j] = g;
-      sh.omega[j] = done ? 1.0 : 0.0;  // per-column done flag for the vote below
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int alldone = 1;
-      for (int j = 0; j < K; j++) alldone &= (sh.omega[j] != 0.0);
-      sh.uni = alldone || it_s >= I.max_inner || its >= I.max_iters;
+      const bool alldone = __all(done);
+      if (threadIdx.x == 0) sh.uni = alldone || it_s >= I.max_inner || its >= I.max_iters;
     }
     __syncthreads();
     if (sh.uni) break;
@@ -1448,22 +1459,24 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     if (have_rhs) phase_install<K, TI, C>(I, row_lo, row_hi);
     const bool hready = have_rhs && I.hvec;  // the update above formed hvec
     int sweeps = 0, its = 0;
-    while (true) {
-      if (threadIdx.x == 0) {
-        int any = 0;
-        for (int j = 0; j < K; j++) any |= sh.active[j];
-        sh.uni = any && sweeps < I.max_sweeps && its < I.max_iters;
-      }
-      __syncthreads();
-      if (!sh.uni) break;
-      if (threadIdx.x < K) {
+    // Whether a sweep follows, and its inner threshold, are decided in wave 0
+    // (one lane per column) together with the state they depend on: here for
+    // the first sweep, after the last residual pass of a sweep for the next.
+    auto next_sweep = [&](int sweeps_done) {
+      const bool act = threadIdx.x < K && sh.active[threadIdx.x];
+      const bool go = __any(act) && sweeps_done < I.max_sweeps && its < I.max_iters;
+      if (go && threadIdx.x < K) {
         // FP32 sweeps reduce the residual by inner_tol; an FP64 inner
         // solve goes straight to the final tolerance.
         const double rs = sh.ithr[threadIdx.x];
         sh.ithr[threadIdx.x] = (sizeof(TI) == 4) ? fmax(I.inner_tol2 * rs, sh.thr[threadIdx.x])
                                                  : sh.thr[threadIdx.x];
       }
-      __syncthreads();
+      if (threadIdx.x == 0) sh.uni = go;
+    };
+    if (threadIdx.x < kWave) next_sweep(0);
+    __syncthreads();
+    while (sh.uni) {
       bool ok;
       if constexpr (KR == 1) ok = krylov_bicgstab<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached);
       else if constexpr (KR == 2) ok = krylov_cg_classic<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
@@ -1474,29 +1487,34 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       // correction and takes the residual of x alone.
       for (int pass = 0; pass < 2; pass++) {
         double acc[2] = {0.0, 0.0};
-        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc, lvn, fuse_next ? I.hpre : nullptr, hready);
+        phase_res<K, TI, C>(I, row_lo, row_hi, r0, sh, acc, lvn, fuse_next ? I.hpre : nullptr, hready,
+                            cached ? &RC : nullptr);
         if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
-        if (threadIdx.x == 0) {
-          int redo = 0;
-          for (int j = 0; j < K; j++) {
+        if (threadIdx.x < kWave) {
+          const int j = threadIdx.x;
+          bool redo_j = false;
+          if (j < K) {
             const bool bad = !(sh.red[j] <= DBL_MAX);  // NaN or infinite
-            redo |= bad && !sh.reset[j];
+            redo_j = bad && !sh.reset[j];
             sh.reset[j] = bad;
           }
-          sh.uni = redo;
+          const bool redo = __any(redo_j) && pass == 0;
+          if (!redo) {  // the sweep's last pass: its state, then the next sweep's decision
+            if (j < K) {
+              sh.reset[j] = 0;
+              const double rr = sh.red[j];
+              sh.active[j] = (sh.bb[j] != 0.0) && (rr >= sh.thr[j]) && (rr == rr);
+              sh.gamma[j] = sh.red[K + j];
+              sh.ithr[j] = rr;
+            }
+            next_sweep(sweeps + 1);
+          }
+          if (j == 0) sh.redo = redo;
         }
         __syncthreads();
-        if (!sh.uni) break;
+        if (!sh.redo) break;
       }
-      if (threadIdx.x < K) sh.reset[threadIdx.x] = 0;
       sweeps++;
-      if (threadIdx.x < K) {
-        const double rr = sh.red[threadIdx.x];
-        sh.active[threadIdx.x] = (sh.bb[threadIdx.x] != 0.0) && (rr >= sh.thr[threadIdx.x]) && (rr == rr);
-        sh.gamma[threadIdx.x] = sh.red[K + threadIdx.x];
-        sh.ithr[threadIdx.x] = rr;
-      }
-      __syncthreads();
     }
     // ---- U: state update
     int notconv = 0;
