@@ -141,7 +141,7 @@ __device__ inline bool has_tag(double v, unsigned e) {
 // reducing fewer values pads with tagged zeros: otherwise entries last
 // written eight epochs earlier (same buffer, same tag) would look current.
 template <int K>
-constexpr int kPadVals = 3 * K;
+constexpr int kPadVals = 4 * K;
 
 // Per-lane partial sums -> this workgroup's share at q*K + j.  A one-workgroup
 // team keeps it in LDS (sh.red).  Larger teams first wait until every wave's
@@ -359,8 +359,9 @@ __device__ inline A row_gather(int lo, int hi, A zero, F f) {
 // Entries in flight for gathers that load five or more values per entry.
 constexpr int kUWide = 4;
 
-template <int K, typename TI, int C>
-__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step, double (&acc)[3]) {
+template <int K, typename TI, int C, int NA>
+__device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step, double (&acc)[NA]) {
+  static_assert(NA >= 3, "phase_rhs sums three values");
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -520,12 +521,12 @@ __device__ __attribute__((always_inline)) inline D3 res_gather_fused(const Inst&
 
 // FP64 residual of x + xi into r (inner precision); acc: <r, r>, <r, D^-1 r>.
 // hpre != nullptr: the next step's update is fused with its right-hand side
-// and takes, should this pass turn out to be the last of the step, the
-// products of the candidate solution from here instead of gathering it:
-// M (x + xi) into xnext and the S-part of the next initial residual,
-// w0 S (x + xi) + S h, into hpre (lvn: the next step's predictor order;
-// hready: the update of this step formed h in hvec; RC: the workgroup's
-// rows are in the LDS row cache).
+// and takes, should this pass turn out to be the last of the step, what it
+// needs from here instead of gathering the candidate solution c = x + xi:
+// the next right-hand side b = M c / dt (+ the Gray-Scott reaction at c)
+// into xnext and the next initial residual b - (w0 S c + S h) into hpre
+// (lvn: the next step's predictor order; hready: the update of this step
+// formed h in hvec; RC: the workgroup's rows are in the LDS row cache).
 template <int K, typename TI, int C>
 __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, int row_lo, int row_hi, TI* r,
                                                                 const Shared& sh, double (&acc)[2], int lvn = 0,
@@ -540,6 +541,7 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
   // Eigen: rhs == 0 -> x = 0, so the next update sees a zero solution
   const bool zero = sh.bb[jr] == 0.0;
   const bool warm = I.warm != 0;
+  const double inv_dt = 1.0 / I.dt;
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
@@ -555,6 +557,7 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
     double sx;
     if (hpre) {
       const int lo = rp[row], hi = rp[row + 1];
+      const double xc_own = ld<C>(I.x + o) + (drop ? 0.0 : (double)ld<C>(xi + o));
       const int* cols = RC ? &RC->col[q][0][threadIdx.x] : nullptr;
       const D3 g =
           !want_h  ? res_gather_fused<0, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols)
@@ -563,8 +566,28 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
                     : res_gather_fused<1, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols);
       sx = g.a;
       const double su = zero ? 0.0 : g.a;
-      gp(hpre)[o] = warm ? w0 * su + (want_h ? g.c : 0.0) : 0.0;
-      gp(I.xnext)[o] = zero ? 0.0 : g.b;
+      double bn = (zero ? 0.0 : g.b) * inv_dt;
+      if (I.eq == EQ_GS) {  // CPUSolver.cpp:134,137 at the candidate
+        auto cand = [&](int j) -> double {  // species j of this row
+          const size_t oj = (size_t)row * K + j;
+          return sh.bb[j] == 0.0 ? 0.0 : ld<C>(I.x + oj) + (sh.reset[j] ? 0.0 : (double)ld<C>(xi + oj));
+        };
+        double uu, vv;
+        if constexpr (K == 2) {  // the row's other species is the pair lane
+          const double xo = zero ? 0.0 : xc_own;
+          const double xp = __shfl_xor(xo, 1, kWave);
+          uu = jr == 0 ? xo : xp;
+          vv = jr == 0 ? xp : xo;
+        } else {
+          uu = cand(0);
+          vv = cand(1);
+        }
+        const double uvv = uu * (vv * vv);
+        const double fr = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
+        bn += (I.weighting ? gp(I.lumped)[row] : 1.0) * fr;
+      }
+      gp(I.xnext)[o] = bn;
+      st<C>(hpre + o, bn - (warm ? w0 * su + (want_h ? g.c : 0.0) : 0.0));
     } else {
       sx = row_gather(rp[row], rp[row + 1], 0.0, [&](int s) {
         const size_t oc = (size_t)ci[s] * K + jr;
@@ -1085,12 +1108,15 @@ __device__ void phase_update(const Inst& I, int row_lo, int row_hi, const Shared
 // initial guess in xnext, which phase_install moves into x (xi = 0) after the
 // barrier.  The history part of S x0 comes from hpre (phase_res or
 // phase_hpre of this step); lvn is the next step's predictor order.  pre:
-// the last residual pass of this step already formed M x_n (in xnext) and
-// all of S x0 (in hpre), so no neighbour is gathered.  lvh > 0: also the
-// history part of the initial guess after that into hvec.
+// the last residual pass of this step already formed b (in xnext) and the
+// initial residual r (in hpre), so no neighbour state is gathered.  lvh > 0:
+// also the history part of the initial guess after that into hvec.
+// RCi (with pre; pipelined CG on cached rows): also the start of that CG,
+// w = S D^-1 r into wq from the neighbours' r in hpre, so the sweep needs no
+// start phase of its own; acc[3] then sums <D^-1 r, w>.
 template <int K, typename TI, int C>
 __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, const Shared& sh, int* flags, int step,
-                                 int lvn, int lvh, bool pre, double (&acc)[3]) {
+                                 int lvn, int lvh, bool pre, const RowCache<TI>* RCi, TI* wq, double (&acc)[4]) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -1114,7 +1140,8 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
   auto xn_at = [&](size_t idx, int j) -> double {
     return sh.bb[j] == 0.0 ? 0.0 : ld<C>(I.x + idx) + (double)ld<C>(xi + idx);
   };
-  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+  int q = 0;  // pass over the workgroup's rows (row cache slot)
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP, q++) {
     const size_t o = (size_t)row * K + jr;
     // every load of the row first: loads and stores share one completion
     // counter, so a load issued after a store also waits for the store
@@ -1124,29 +1151,36 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
     const double vo = I.extrap >= 3 ? ld<C>(I.v + o) : 0.0;
     const double ho = I.extrap >= 4 ? ld<C>(I.h3 + o) : 0.0;
     // ---- next right-hand side: b = (M/dt) x_n (+ Gray-Scott reaction), r = b - S x0
-    double mx, sx;  // M x_n and S x0
+    double b, r64;
+    TI wa = (TI)0;  // S D^-1 r (with RCi)
     if (pre) {      // both formed by the last residual pass of this step
-      mx = gp(I.xnext)[o];
-      sx = gp(I.hpre)[o];
+      b = gp(I.xnext)[o];
+      r64 = ld<C>(I.hpre + o);
+      if (RCi) {
+        TI g[kCacheE];
+#pragma unroll
+        for (int e = 0; e < kCacheE; e++)
+          g[e] = to_ti<TI>(ld<C>(I.hpre + (size_t)RCi->col[q][e][threadIdx.x] * K + jr));
+#pragma unroll
+        for (int e = 0; e < kCacheE; e++) wa += RCi->val[q][e][threadIdx.x] * g[e];
+      }
     } else {
       const D2 g = row_gather(rp[row], rp[row + 1], D2{0.0, 0.0}, [&](int s) {
         const double xc = xn_at((size_t)ci[s] * K + jr, jr);
         return D2{M64[s] * xc, warm ? S64[(size_t)s * kv + vj] * xc : 0.0};
       });
-      mx = g.a;
-      sx = warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0;
-    }
-    double b = mx * inv_dt;
-    if (I.eq == EQ_GS) {  // CPUSolver.cpp:134,137 with the new state
-      const double uu = xn_at((size_t)row * K, 0), vv = xn_at((size_t)row * K + 1, 1);
-      const double uvv = uu * (vv * vv);
-      const double fr = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
-      const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
-      b += w * fr;
+      b = g.a * inv_dt;
+      if (I.eq == EQ_GS) {  // CPUSolver.cpp:134,137 with the new state
+        const double uu = xn_at((size_t)row * K, 0), vv = xn_at((size_t)row * K + 1, 1);
+        const double uvv = uu * (vv * vv);
+        const double fr = (jr == 0) ? (-uvv + I.p3 * (1.0 - uu)) : (uvv - (I.p3 + I.p4) * vv);
+        const double w = I.weighting ? gp(I.lumped)[row] : 1.0;
+        b += w * fr;
+      }
+      r64 = b - (warm ? w0 * g.b + (lvn > 0 ? gp(I.hpre)[o] : 0.0) : 0.0);
     }
     const TI dj = dinv[(size_t)row * kv + vj];
     const double x0 = warm ? w0 * xn + w1 * uo + w2 * ao + w3 * vo + w4 * ho : 0.0;
-    const double r64 = b - sx;
     const TI rr = to_ti<TI>(r64);
     // ---- update: state and history x_{n-3} <- x_{n-2} <- x_{n-1} <- x_n (own row)
     if (I.extrap >= 4) st<C>(I.h4 + o, ho);
@@ -1162,9 +1196,12 @@ __device__ void phase_update_rhs(const Inst& I, int row_lo, int row_hi, TI* r, c
     gp(I.b)[o] = b;
     gp(I.xnext)[o] = x0;
     st<C>(r + o, rr);
+    if (RCi) st<C>(wq + o, wa);
     acc[0] += b * b;
     acc[1] += r64 * r64;
-    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)(dj * rr);
+    const TI uz = dj * rr;
+    acc[2] += (I.krylov == 1) ? (double)rr * (double)rr : (double)rr * (double)uz;
+    if (RCi) acc[3] += (double)uz * (double)wa;
   }
   if (f) atomicOr(flags, f);
 }
@@ -1216,27 +1253,31 @@ __device__ void phase_hpre(const Inst& I, int row_lo, int row_hi, int lvn, bool 
 // ---------------------------------------------------------------------------
 
 // Pipelined (Chronopoulos-Gear) PCG on the current sweep; r starts in I.r.
+// started: the update phase already formed w = S D^-1 r and the start
+// scalars (x holds the initial guess, xi = 0).
 template <int K, typename TI, int C>
 __device__ bool krylov_pcg(const Inst& I, const Team& T, int rank, unsigned& e, Shared& sh, int row_lo, int row_hi,
-                           int& its, const RowCache<TI>& RC, bool cached) {
+                           int& its, const RowCache<TI>& RC, bool cached, bool started) {
   TI* rb[2] = {(TI*)I.r, (TI*)I.z};
   TI* wb[2] = {(TI*)I.q, (TI*)I.rh};
   TI* sb[2] = {(TI*)I.s, (TI*)I.t};
   int par = 0;
-  {
-    double acc[2] = {0.0, 0.0};
-    if (cached) phase_pcg_init_c<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], RC, acc);
-    else phase_pcg_init<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], acc);
-    if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+  if (!started) {
+    {
+      double acc[2] = {0.0, 0.0};
+      if (cached) phase_pcg_init_c<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], RC, acc);
+      else phase_pcg_init<K, TI, C>(I, row_lo, row_hi, rb[0], wb[0], acc);
+      if (!team_reduce<K, 2, C>(acc, T, rank, e, sh)) return false;
+    }
+    if (threadIdx.x < K) {
+      const int j = threadIdx.x;
+      const double g = sh.red[j], d = sh.red[K + j];
+      sh.gamma[j] = g;
+      sh.alpha[j] = (sh.active[j] && d != 0.0) ? g / d : 0.0;
+      sh.beta[j] = 0.0;
+    }
+    __syncthreads();
   }
-  if (threadIdx.x < K) {
-    const int j = threadIdx.x;
-    const double g = sh.red[j], d = sh.red[K + j];
-    sh.gamma[j] = g;
-    sh.alpha[j] = (sh.active[j] && d != 0.0) ? g / d : 0.0;
-    sh.beta[j] = 0.0;
-  }
-  __syncthreads();
   int it_s = 0;
   bool first = true;
   while (true) {
@@ -1436,23 +1477,34 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     const int lvn = (fuse_next && I.warm) ? min(I.extrap, I.hist + step + 1) : 0;  // next step's predictor order
     // ---- R: right-hand side and initial residual (after the previous step's
     // update, in the same phase, when that was deferred)
+    // the pipelined CG of the first sweep starts in the update phase (from
+    // the initial residuals the last residual pass stored)
+    const bool started = KR == 0 && have_rhs && rhs_pre && cached;
     {
-      double acc[3] = {0.0, 0.0, 0.0};
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
       if (have_rhs)
         phase_update_rhs<K, TI, C>(I, row_lo, row_hi, r0, sh, I.status, step - 1,
-                                   I.warm ? min(I.extrap, I.hist + step) : 0, I.hvec ? lvn : 0, rhs_pre, acc);
+                                   I.warm ? min(I.extrap, I.hist + step) : 0, I.hvec ? lvn : 0, rhs_pre,
+                                   started ? &RC : nullptr, (TI*)I.q, acc);
       else
         phase_rhs<K, TI, C>(I, row_lo, row_hi, r0, step, acc);
-      if (!team_reduce<K, 3, C>(acc, T, rank, e, sh)) return false;
+      if (!team_reduce<K, 4, C>(acc, T, rank, e, sh)) return false;
     }
     if (threadIdx.x < K) {
-      const double bb = sh.red[threadIdx.x], rr = sh.red[K + threadIdx.x];
-      sh.bb[threadIdx.x] = bb;
-      sh.thr[threadIdx.x] = fmax(I.tol2 * bb, DBL_MIN);
-      sh.active[threadIdx.x] = (bb != 0.0) && (rr >= sh.thr[threadIdx.x]);
-      sh.gamma[threadIdx.x] = sh.red[2 * K + threadIdx.x];
-      sh.ithr[threadIdx.x] = rr;  // residual^2 at sweep start
-      sh.reset[threadIdx.x] = 0;
+      const int j = threadIdx.x;
+      const double bb = sh.red[j], rr = sh.red[K + j], g = sh.red[2 * K + j];
+      sh.bb[j] = bb;
+      sh.thr[j] = fmax(I.tol2 * bb, DBL_MIN);
+      const bool act = (bb != 0.0) && (rr >= sh.thr[j]);
+      sh.active[j] = act;
+      sh.gamma[j] = g;
+      sh.ithr[j] = rr;  // residual^2 at sweep start
+      sh.reset[j] = 0;
+      if (started) {  // as krylov_pcg after its start phase
+        const double d = sh.red[3 * K + j];
+        sh.alpha[j] = (act && d != 0.0) ? g / d : 0.0;
+        sh.beta[j] = 0.0;
+      }
     }
     __syncthreads();
     // own rows only: the phases up to the next barrier read no other row's x
@@ -1480,7 +1532,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
       bool ok;
       if constexpr (KR == 1) ok = krylov_bicgstab<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached);
       else if constexpr (KR == 2) ok = krylov_cg_classic<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its);
-      else ok = krylov_pcg<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached);
+      else ok = krylov_pcg<K, TI, C>(I, T, rank, e, sh, row_lo, row_hi, its, RC, cached, started && sweeps == 0);
       if (!ok) return false;
       // ---- Res: FP64 residual of x + xi.  A column whose residual is not
       // finite (the sweep broke down in the inner precision) drops its
