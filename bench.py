@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # against a plain copy kernel) for the dominant kernel of a workload; the
 # summaries with their commands are in profiles/.
 PROFILED_TRAFFIC = {
-    "grayscott": {"bytes_per_step": 3.8244e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
+    "grayscott": {"bytes_per_step": 3.9685e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
     "spmv": {"bytes_per_launch": 1.2087e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
 }
 METRIC = "DOF·steps/s at rel-L2≤1e-8 vs FP64"
