@@ -547,6 +547,55 @@ def test_stream_engine_multi_rhs(gpu, eq, krylov, k, prec):
     b.close()
 
 
+@pytest.mark.parametrize("precision", [_lib.FEM_PREC_FP32_IR, _lib.FEM_PREC_FP64])
+def test_fused_steps_with_and_without_sweeps(gpu, precision):
+    """Inside one launch, a step whose initial guess already meets the
+    tolerance runs no refinement sweep; the next step's update then forms
+    its right-hand side by gathering the solution, while after a sweep it
+    takes b and r from the last residual pass (and the pipelined CG starts
+    in that phase).  Heat under Neumann conditions from 1 + 1e-2 noise,
+    with the previous solution as the initial guess: the fast modes die out
+    within a few steps, after which that guess of the nearly constant state
+    meets a loose tolerance, so one launch runs both kinds of step.  (An
+    extrapolating guess amplifies the tolerance-level error of its history
+    and nearly always needs a sweep.)  The launch must match single-step
+    launches, which form every right-hand side from the state, and the
+    oracle."""
+    print("case fused sweeps mix", precision, flush=True)
+    m = meshes.jittered_square(40, seed=8)
+    s = oracle.System(m, bc=1)
+    u0 = 1.0 + 1e-2 * np.random.default_rng(3).uniform(-1.0, 1.0, s.n)
+    nsteps = 24
+
+    def run(tol, split):
+        sol = make_solver(FEMContext(m, 1), Equation.Heat, HeatParameters(1e-3, 1.0), precision=precision, tol=tol,
+                          predictor=_lib.FEM_PREDICTOR_PREVIOUS)
+        sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+        sweeps = []
+        for _ in range(nsteps if split else 1):
+            assert sol.advance_time(1 if split else nsteps) == _lib.FEM_OK
+            sweeps.append(sol.last_stats.sweeps)
+        return sol.get_state_unknowns(), sweeps
+
+    # per-step sweeps of single-step launches (same initial guesses as one launch)
+    for tol in (2e-5, 1e-5, 3e-5, 5e-6, 5e-5):
+        steps1, sweeps = run(tol, True)
+        print("tol", tol, "sweeps per step", sweeps, flush=True)
+        if 0 in sweeps[1:-1] and any(sweeps[1:-1]):
+            break
+    else:
+        pytest.fail("no tolerance gave steps with and without sweeps")
+    whole, wsweeps = run(tol, False)
+    print("one launch: sweeps", wsweeps, flush=True)
+    assert 0 < wsweeps[0] < nsteps
+    # the perturbation about 1, which the solves resolve to tol |b| (|b| ~ |M 1| / dt)
+    assert rel(whole - 1.0, steps1 - 1.0) < 1e-3
+    # the oracle's CG from the same guess with the same stopping rule: it also
+    # stops at once where the guess meets the tolerance
+    _, ref, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], nsteps, u0.copy(), tol=tol, warm=True)
+    assert rel(whole, ref) < 10 * tol
+
+
 @pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC, _lib.FEM_PREDICTOR_CUBIC,
                                   _lib.FEM_PREDICTOR_QUARTIC])
 @pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream",
