@@ -557,7 +557,6 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
     double sx;
     if (hpre) {
       const int lo = rp[row], hi = rp[row + 1];
-      const double xc_own = ld<C>(I.x + o) + (drop ? 0.0 : (double)ld<C>(xi + o));
       const int* cols = RC ? &RC->col[q][0][threadIdx.x] : nullptr;
       const D3 g =
           !want_h  ? res_gather_fused<0, false, K, TI, C>(I, lo, hi, jr, vj, drop, lvn, w1, w2, w3, w4, cols)
@@ -574,7 +573,7 @@ __device__ __attribute__((always_inline)) inline void phase_res(const Inst& I, i
         };
         double uu, vv;
         if constexpr (K == 2) {  // the row's other species is the pair lane
-          const double xo = zero ? 0.0 : xc_own;
+          const double xo = cand(jr);
           const double xp = __shfl_xor(xo, 1, kWave);
           uu = jr == 0 ? xo : xp;
           vv = jr == 0 ? xp : xo;
