@@ -394,12 +394,12 @@ def run_wave_c3(args):
     return out
 
 
-def c4_instances(ninst=256):
-    """The 256 batched instances: side = round(exp(U(ln 32, ln 512))),
-    kappa = exp(U(ln 1e-4, ln 1e-1)), v = U(0,1) (cos t, sin t, 0) (seed 4);
-    jittered Delaunay-split lattices (seed 1000 + i); 16 Gaussian bumps per
-    mesh, centre U[0.2,0.8]^2 and sigma U(0.02, 0.1) (seed 5)."""
-    from femhip import meshes
+def c4_specs(ninst=256):
+    """Parameters of the batched instances, without their meshes: side =
+    round(exp(U(ln 32, ln 512))), kappa = exp(U(ln 1e-4, ln 1e-1)), v = U(0,1)
+    (cos t, sin t, 0) (seed 4); 16 Gaussian bumps per mesh, centre U[0.2,0.8]^2
+    and sigma U(0.02, 0.1) (seed 5).  Instance i's mesh is the jittered
+    Delaunay-split lattice of that side with seed 1000 + i."""
     r4 = np.random.Generator(np.random.PCG64(4))
     r5 = np.random.Generator(np.random.PCG64(5))
     out = []
@@ -409,9 +409,17 @@ def c4_instances(ninst=256):
         sp, th = r4.uniform(0, 1), r4.uniform(0, 2 * np.pi)
         cen = r5.uniform(0.2, 0.8, size=(16, 2))
         sig = r5.uniform(0.02, 0.1, size=16)
-        out.append((meshes.jittered_square(side, seed=1000 + i), kap, (sp * np.cos(th), sp * np.sin(th), 0.0),
-                    cen, sig))
+        out.append((side, kap, (sp * np.cos(th), sp * np.sin(th), 0.0), cen, sig))
     return out
+
+
+def c4_instances(ninst=256, which=None):
+    """The batched instances (c4_specs) with their meshes: (mesh, kappa,
+    velocity, centres, sigmas) per instance; `which` selects indices."""
+    from femhip import meshes
+    specs = c4_specs(ninst)
+    idx = range(ninst) if which is None else which
+    return [(meshes.jittered_square(specs[i][0], seed=1000 + i),) + specs[i][1:] for i in idx]
 
 
 def run_batched_c4(args):

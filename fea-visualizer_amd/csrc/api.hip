@@ -187,53 +187,26 @@ int fem_solver_set_equation(fem_solver* s, int32_t equation, const void* params,
   })
 }
 
-// Surface-space state: gather nv -> N through idx_map (CPUSolver.cpp:13-21).
+// Surface-space state: gather nv -> N through idx_map (CPUSolver.cpp:13-21)
+// on the device; entries equal to the state rounded to float are kept.
 int fem_solver_set_state(fem_solver* s, int32_t field, const float* values_nv) {
   FEM_GUARD({
     if (!s || !values_nv) {
       set_error("null argument");
       return FEM_ERR_INVALID;
     }
-    fem_batch* b = s->batch;
-    fem_ctx* c = b->insts[0].ctx;
-    double* d;
-    int col, nc;
-    int rc = batch_field_ptr(b, 0, field, &d, &col, &nc);
-    if (rc) return rc;
-    std::vector<int> idx(c->nv);
-    rc = fem_ctx_get_idx_map(c, idx.data());
-    if (rc) return rc;
-    std::vector<double> x((size_t)c->n * nc, 0.0);
-    for (int i = 0; i < c->nv; i++)
-      if (idx[i] >= 0)
-        for (int j = 0; j < nc; j++) x[(size_t)idx[i] * nc + j] = values_nv[(size_t)i * nc + j];
-    return batch_set_state_unknowns(b, 0, field, x.data());
+    return batch_set_surface(s->batch, 0, field, values_nv);
   })
 }
 
-// Scatter N -> nv, boundary entries written as 0 (CPUSolver.cpp:28-37).
+// Scatter N -> nv on the device, boundary entries written as 0 (CPUSolver.cpp:28-37).
 int fem_solver_get_state(fem_solver* s, int32_t field, float* values_nv) {
   FEM_GUARD({
     if (!s || !values_nv) {
       set_error("null argument");
       return FEM_ERR_INVALID;
     }
-    fem_batch* b = s->batch;
-    fem_ctx* c = b->insts[0].ctx;
-    double* d;
-    int col, nc;
-    int rc = batch_field_ptr(b, 0, field, &d, &col, &nc);
-    if (rc) return rc;
-    std::vector<int> idx(c->nv);
-    rc = fem_ctx_get_idx_map(c, idx.data());
-    if (rc) return rc;
-    std::vector<double> x((size_t)c->n * nc, 0.0);
-    rc = batch_get_state_unknowns(b, 0, field, x.data());
-    if (rc) return rc;
-    for (int i = 0; i < c->nv; i++)
-      for (int j = 0; j < nc; j++)
-        values_nv[(size_t)i * nc + j] = idx[i] >= 0 ? (float)x[(size_t)idx[i] * nc + j] : 0.0f;
-    return FEM_OK;
+    return batch_get_surface(s->batch, 0, field, values_nv);
   })
 }
 
@@ -412,6 +385,16 @@ int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
       return FEM_ERR_INVALID;
     }
     return batch_graph_launch(b, stats);
+  })
+}
+
+int fem_batch_plan_info(fem_batch* b, fem_plan_info* info) {
+  FEM_GUARD({
+    if (!b || !info) {
+      set_error("null batch or info");
+      return FEM_ERR_INVALID;
+    }
+    return batch_plan_info(b, info);
   })
 }
 

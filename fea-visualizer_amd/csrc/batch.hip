@@ -99,7 +99,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 void free_inst(InstHost& h) {
   void* ptrs[] = {h.d_S64, h.d_Si,    h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v,   h.d_a,
                   h.d_h3,  h.d_h4,    h.d_xnext,  h.d_hpre,  h.d_hvec, h.d_x, h.d_b, h.d_vec,
-                  h.d_status, h.d_counters, h.d_last_res};
+                  h.d_surf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   fem_ctx* c = h.ctx;
@@ -129,10 +129,12 @@ int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
   h.d_x = dalloc<double>((size_t)n * k, &rc);
   h.d_b = dalloc<double>((size_t)n * k, &rc);
   h.d_vec = (void*)dalloc<char>((size_t)n * k * tb * 11, &rc);
-  h.d_status = dalloc<int>(4, &rc);
-  h.d_counters = dalloc<long long>(4, &rc);
-  h.d_last_res = dalloc<double>(kMaxK, &rc);
+  h.d_surf = dalloc<float>((size_t)h.ctx->nv * k, &rc);
   if (rc) return rc;
+  char* info = b->d_info + (size_t)(&h - b->insts.data()) * kInfoBytes;
+  h.d_counters = (long long*)info;
+  h.d_status = (int*)(info + kInfoStatus);
+  h.d_last_res = (double*)(info + kInfoLastRes);
   const size_t vb = (size_t)n * k * sizeof(double);
   FEM_HIP(hipMemsetAsync(h.d_u, 0, vb, b->stream));
   FEM_HIP(hipMemsetAsync(h.d_v, 0, vb, b->stream));
@@ -286,7 +288,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
 }
 
 void free_plan(fem_batch* b) {
-  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_slots, b->d_scal, b->d_abort};
+  void* ptrs[] = {b->d_insts, b->d_teams, b->d_wg_team, b->d_team_inst, b->d_slots, b->d_scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b->d_insts = nullptr;
@@ -295,7 +297,7 @@ void free_plan(fem_batch* b) {
   b->d_team_inst = nullptr;
   b->d_slots = nullptr;
   b->d_scal = nullptr;
-  b->d_abort = nullptr;
+  b->uploaded.clear();
   b->plan_ready = false;
 }
 
@@ -320,10 +322,13 @@ int build_plan(fem_batch* b) {
       return FEM_ERR_INVALID;
     }
   const int inner_fp32 = inner_bytes(b) == 4;
-  int maxb = 0;
-  int rc = step_max_coresident(k, inner_fp32, kr, 1, &maxb);
+  // co-resident workgroups of each variant: plain (one-workgroup teams),
+  // coherent and fenced exchange; they differ in registers
+  int maxv[3] = {0, 0, 0};
+  int rc = 0;
+  for (int m = 0; m < 3 && !rc; m++) rc = step_max_coresident(k, inner_fp32, kr, m, &maxv[m]);
   if (rc) return rc;
-  if (maxb <= 0) {
+  if (std::min(maxv[0], std::min(maxv[1], maxv[2])) <= 0) {
     set_error("persistent kernel cannot be resident on this device");
     return FEM_ERR_HIP;
   }
@@ -335,55 +340,95 @@ int build_plan(fem_batch* b) {
   }
   std::vector<std::vector<int>> teams;  // instance lists
   std::vector<int> sizes;
-  if (b->opts.team_size > 0 && (long long)b->opts.team_size * ninst <= maxb) {
-    for (int i = 0; i < ninst; i++) {
-      teams.push_back({i});
-      sizes.push_back(b->opts.team_size);
-    }
-  } else if (ninst == 1) {
-    const long long lanes = (long long)b->insts[0].ctx->n * k;
-    int g = (int)std::max(1LL, (lanes + kBlock - 1) / kBlock);  // ~one row per lane
-    g = std::min(g, std::min(maxb, 1024));
-    teams.push_back({0});
-    sizes.push_back(g);
-  } else {
-    const int budget = (b->grid_budget > 0) ? std::min(b->grid_budget, maxb) : maxb;
-    const double share = W / budget;
-    std::vector<int> small;
-    int used = 0;
-    for (int i = 0; i < ninst; i++) {
-      int g = (int)std::floor(work[i] / share);
-      if (g >= 2) {
+  // Plan within `maxb` co-resident workgroups.
+  auto make_plan = [&](int maxb) {
+    teams.clear();
+    sizes.clear();
+    if (b->opts.team_size > 0 && (long long)b->opts.team_size * ninst <= maxb) {
+      for (int i = 0; i < ninst; i++) {
         teams.push_back({i});
-        sizes.push_back(g);
-        used += g;
-      } else {
-        small.push_back(i);
+        sizes.push_back(b->opts.team_size);
+      }
+    } else if (ninst == 1) {
+      const long long lanes = (long long)b->insts[0].ctx->n * k;
+      int g = (int)std::max(1LL, (lanes + kBlock - 1) / kBlock);  // ~one row per lane
+      g = std::min(g, std::min(maxb, 1024));
+      teams.push_back({0});
+      sizes.push_back(g);
+    } else {
+      const int budget = (b->grid_budget > 0) ? std::min(b->grid_budget, maxb) : maxb;
+      const double share = W / budget;
+      std::vector<int> small;
+      int used = 0;
+      for (int i = 0; i < ninst; i++) {
+        int g = (int)std::floor(work[i] / share);
+        if (g >= 2) {
+          teams.push_back({i});
+          sizes.push_back(g);
+          used += g;
+        } else {
+          small.push_back(i);
+        }
+      }
+      if (used > budget) {  // rounding guard
+        double f = (double)budget / (used + (double)small.size());
+        used = 0;
+        for (auto& g : sizes) {
+          g = std::max(1, (int)(g * f));
+          used += g;
+        }
+      }
+      int bins = std::max(1, std::min((int)small.size(), std::max(1, budget - used)));
+      std::sort(small.begin(), small.end(), [&](int x, int y) { return work[x] > work[y]; });
+      std::vector<double> load(bins, 0.0);
+      std::vector<std::vector<int>> lists(bins);
+      for (int i : small) {
+        int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[m].push_back(i);
+        load[m] += work[i];
+      }
+      for (int m = 0; m < bins; m++)
+        if (!lists[m].empty()) {
+          teams.push_back(lists[m]);
+          sizes.push_back(1);
+        }
+    }
+  };
+  // Exchange mode of a plan: a workgroup of a multi-workgroup team whose rows
+  // exceed two per lane (the LDS row cache) streams its phases: fenced
+  // barriers.  Plans of one-workgroup teams run the plain variant.
+  const int forced = (b->opts.exchange == FEM_EXCHANGE_COHERENT || b->opts.exchange == FEM_EXCHANGE_FENCED)
+                         ? b->opts.exchange : 0;
+  auto plan_mode = [&]() {
+    bool multi = false, streaming = false;
+    for (size_t t = 0; t < teams.size(); t++) {
+      if (sizes[t] <= 1) continue;
+      multi = true;
+      for (int i : teams[t]) {
+        const long long rows = (b->insts[i].ctx->n + sizes[t] - 1) / sizes[t];
+        if (rows * k > 2LL * kBlock) streaming = true;
       }
     }
-    if (used > budget) {  // rounding guard
-      double f = (double)budget / (used + (double)small.size());
-      used = 0;
-      for (auto& g : sizes) {
-        g = std::max(1, (int)(g * f));
-        used += g;
-      }
-    }
-    int bins = std::max(1, std::min((int)small.size(), std::max(1, budget - used)));
-    std::sort(small.begin(), small.end(), [&](int x, int y) { return work[x] > work[y]; });
-    std::vector<double> load(bins, 0.0);
-    std::vector<std::vector<int>> lists(bins);
-    for (int i : small) {
-      int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      lists[m].push_back(i);
-      load[m] += work[i];
-    }
-    for (int m = 0; m < bins; m++)
-      if (!lists[m].empty()) {
-        teams.push_back(lists[m]);
-        sizes.push_back(1);
-      }
+    return !multi ? 0
+           : forced ? forced
+           : b->mode_override ? b->mode_override
+           : streaming ? kExchangeFenced : kExchangeCoherent;
+  };
+  auto plan_grid = [&]() {
+    int g = 0;
+    for (int s_ : sizes) g += s_;
+    return g;
+  };
+  // Size for the coherent variant first; if the variant the plan then picks
+  // holds fewer workgroups, re-plan within the smallest limit of all three,
+  // which any mode fits.
+  make_plan(maxv[kExchangeCoherent]);
+  int mode = plan_mode();
+  if (plan_grid() > maxv[mode]) {
+    make_plan(std::min(maxv[0], std::min(maxv[1], maxv[2])));
+    mode = plan_mode();
   }
+  const int maxb = maxv[mode];
   const int nteams = (int)teams.size();
   int grid = 0;
   for (int g : sizes) grid += g;
@@ -408,7 +453,6 @@ int build_plan(fem_batch* b) {
   b->d_team_inst = dalloc<int>(team_inst.size(), &rc);
   b->d_slots = dalloc<Slot>(grid, &rc);
   b->d_scal = dalloc<double>((size_t)nteams * kSlotBufs * kMaxRed, &rc);
-  b->d_abort = dalloc<int>(1, &rc);
   if (rc) return rc;
   int poff = 0;
   for (int t = 0; t < nteams; t++) {
@@ -425,6 +469,7 @@ int build_plan(fem_batch* b) {
   std::vector<Inst> hi(ninst);
   for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
+  b->uploaded = hi;
   FEM_HIP(hipMemcpyAsync(b->d_teams, ht.data(), sizeof(Team) * nteams, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_wg_team, wg_team.data(), sizeof(int) * grid, hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipMemcpyAsync(b->d_team_inst, team_inst.data(), sizeof(int) * team_inst.size(), hipMemcpyHostToDevice,
@@ -438,23 +483,7 @@ int build_plan(fem_batch* b) {
   b->launch.k = k;
   b->launch.inner_fp32 = inner_fp32;
   b->launch.kr = kr;
-  // exchange mode: a workgroup of a multi-workgroup team whose rows exceed
-  // two per lane (the LDS row cache) streams its phases: fenced barriers
-  bool multi = false, streaming = false;
-  for (int t = 0; t < nteams; t++) {
-    if (sizes[t] <= 1) continue;
-    multi = true;
-    for (int i : teams[t]) {
-      const long long rows = (b->insts[i].ctx->n + sizes[t] - 1) / sizes[t];
-      if (rows * k > 2LL * kBlock) streaming = true;
-    }
-  }
-  const int forced = (b->opts.exchange == FEM_EXCHANGE_COHERENT || b->opts.exchange == FEM_EXCHANGE_FENCED)
-                         ? b->opts.exchange : 0;
-  b->launch.coh = !multi ? 0
-                  : forced ? forced
-                  : b->mode_override ? b->mode_override
-                  : streaming ? kExchangeFenced : kExchangeCoherent;
+  b->launch.coh = mode;
   b->nteams = nteams;
   b->grid = grid;
   b->team_sizes = sizes;
@@ -476,6 +505,8 @@ void batch_free(fem_batch* b) {
   if (b->d_sinsts) (void)hipFree(b->d_sinsts);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->d_info) (void)hipFree(b->d_info);
+  if (b->h_info) (void)hipHostFree(b->h_info);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -515,6 +546,27 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
     return hip_fail(e, "batch_create");
   }
   b->insts.resize(count);
+  // outcome records of every instance, then the abort and surface-change
+  // flags: one copy brings all of them back after a launch
+  b->info_bytes = (size_t)count * kInfoBytes + 2 * sizeof(int);
+  {
+    int rc = 0;
+    b->d_info = dalloc<char>(b->info_bytes, &rc);
+    if (!rc) {
+      e = hipHostMalloc((void**)&b->h_info, b->info_bytes, hipHostMallocDefault);
+      if (e != hipSuccess) rc = hip_fail(e, "batch_create: hipHostMalloc");
+    }
+    if (!rc) {
+      e = hipMemsetAsync(b->d_info, 0, b->info_bytes, b->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "batch_create: hipMemsetAsync");
+    }
+    if (rc) {
+      batch_free(b);
+      return rc;
+    }
+    b->d_abort = (int*)(b->d_info + (size_t)count * kInfoBytes);
+    b->d_changed = b->d_abort + 1;
+  }
   for (int i = 0; i < count; i++) {
     if (!ctxs[i]) {
       batch_free(b);
@@ -749,12 +801,22 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
   InstHost& h = b->insts[inst];
   const long long n = h.ctx->n;
   if (n == 0) return 0;
+  if (nc == h.k) {  // the field spans every column: copy straight into it
+    FEM_HIP(hipMemcpyAsync(d, x, sizeof(double) * n * nc, hipMemcpyHostToDevice, b->stream));
+    FEM_HIP(hipStreamSynchronize(b->stream));
+    h.hist = 0;  // the predictor's history no longer matches the state
+    return 0;
+  }
   double* tmp = dalloc<double>((size_t)n * nc, &rc);
   if (!tmp) return rc;
-  FEM_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * n * nc, hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
-  FEM_HIP(hipStreamSynchronize(b->stream));
+  hipError_t e = hipMemcpyAsync(tmp, x, sizeof(double) * n * nc, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   (void)hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "set_state_unknowns");
   h.hist = 0;  // the predictor's history no longer matches the state
   return 0;
 }
@@ -767,12 +829,87 @@ int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x) {
   InstHost& h = b->insts[inst];
   const long long n = h.ctx->n;
   if (n == 0) return 0;
+  if (nc == h.k) {
+    FEM_HIP(hipMemcpyAsync(x, d, sizeof(double) * n * nc, hipMemcpyDeviceToHost, b->stream));
+    FEM_HIP(hipStreamSynchronize(b->stream));
+    return 0;
+  }
   double* tmp = dalloc<double>((size_t)n * nc, &rc);
   if (!tmp) return rc;
   hipLaunchKernelGGL(k_get_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
-  FEM_HIP(hipMemcpyAsync(x, tmp, sizeof(double) * n * nc, hipMemcpyDeviceToHost, b->stream));
-  FEM_HIP(hipStreamSynchronize(b->stream));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(x, tmp, sizeof(double) * n * nc, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   (void)hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "get_state_unknowns");
+  return 0;
+}
+
+// Surface values -> state through idx_map (CPUSolver.cpp:13-21).  An entry
+// whose state, rounded to float, already equals the value is left alone: a
+// caller that writes back what get_state returned (the frame loop of
+// Application.cpp:530-540 gathers and scatters every frame) keeps the FP64
+// state and the predictor history.  *changed is set when any entry was
+// written.
+__global__ void k_surface_gather(const int* idx, int nv, const float* surf, int nc, double* state, int k, int col,
+                                 int* changed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int r = idx[i];
+  if (r < 0) return;
+  bool any = false;
+  for (int j = 0; j < nc; j++) {
+    const float v = surf[(size_t)i * nc + j];
+    double* d = state + (size_t)r * k + col + j;
+    if ((float)*d != v) {
+      *d = (double)v;
+      any = true;
+    }
+  }
+  if (any) *changed = 1;
+}
+
+// State -> surface values, boundary vertices 0 (CPUSolver.cpp:28-37).
+__global__ void k_surface_scatter(const int* idx, int nv, const double* state, int k, int col, int nc, float* surf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int r = idx[i];
+  for (int j = 0; j < nc; j++) surf[(size_t)i * nc + j] = r >= 0 ? (float)state[(size_t)r * k + col + j] : 0.0f;
+}
+
+int batch_set_surface(fem_batch* b, int inst, int field, const float* values_nv) {
+  double* d;
+  int col, nc;
+  int rc = batch_field_ptr(b, inst, field, &d, &col, &nc);
+  if (rc) return rc;
+  InstHost& h = b->insts[inst];
+  fem_ctx* c = h.ctx;
+  if (c->nv == 0) return 0;
+  FEM_HIP(hipMemcpyAsync(h.d_surf, values_nv, sizeof(float) * c->nv * nc, hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipMemsetAsync(b->d_changed, 0, sizeof(int), b->stream));
+  hipLaunchKernelGGL(k_surface_gather, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, b->stream, c->d_idx, c->nv,
+                     h.d_surf, nc, d, h.k, col, b->d_changed);
+  FEM_HIP(hipGetLastError());
+  int* hc = (int*)(b->h_info + b->insts.size() * kInfoBytes) + 1;
+  FEM_HIP(hipMemcpyAsync(hc, b->d_changed, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  if (*hc) h.hist = 0;  // the predictor's history no longer matches the state
+  return 0;
+}
+
+int batch_get_surface(fem_batch* b, int inst, int field, float* values_nv) {
+  double* d;
+  int col, nc;
+  int rc = batch_field_ptr(b, inst, field, &d, &col, &nc);
+  if (rc) return rc;
+  InstHost& h = b->insts[inst];
+  fem_ctx* c = h.ctx;
+  if (c->nv == 0) return 0;
+  hipLaunchKernelGGL(k_surface_scatter, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, b->stream, c->d_idx, c->nv,
+                     d, h.k, col, nc, h.d_surf);
+  FEM_HIP(hipGetLastError());
+  FEM_HIP(hipMemcpyAsync(values_nv, h.d_surf, sizeof(float) * c->nv * nc, hipMemcpyDeviceToHost, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
 }
 
@@ -821,26 +958,40 @@ int batch_prepare(fem_batch* b, bool need_plan) {
   if (!need_plan) return 0;
   if (!b->plan_ready) return build_plan(b);
   // descriptors may have changed since the plan was built (trajectory
-  // records, equation override): upload the current ones
+  // records, equation override, predictor history): upload them when they did
+  bool same = b->uploaded.size() == b->insts.size();
+  for (size_t i = 0; same && i < b->insts.size(); i++)
+    same = std::memcmp(&b->uploaded[i], &b->insts[i].dev, sizeof(Inst)) == 0;
+  if (same) return 0;
   std::vector<Inst> hi(b->insts.size());
   for (size_t i = 0; i < hi.size(); i++) hi[i] = b->insts[i].dev;
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * hi.size(), hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipStreamSynchronize(b->stream));
+  b->uploaded = hi;
+  return 0;
+}
+
+// Bring the info block (every instance's record and the flags) back in one
+// copy and fold it into the call's statistics and status.
+int read_info(fem_batch* b) {
+  FEM_HIP(hipMemcpyAsync(b->h_info, b->d_info, b->info_bytes, hipMemcpyDeviceToHost, b->stream));
   FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
 }
 
 int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
+  int rc = read_info(b);
+  if (rc) return rc;
   long long its = 0, sw = 0, steps = 0;
   int status = 0;
   double res = 0.0;
-  for (auto& h : b->insts) {
-    long long cnt[4];
-    int st[4];
-    double lr[kMaxK];
-    FEM_HIP(hipMemcpyAsync(cnt, h.d_counters, sizeof(cnt), hipMemcpyDeviceToHost, b->stream));
-    FEM_HIP(hipMemcpyAsync(st, h.d_status, sizeof(st), hipMemcpyDeviceToHost, b->stream));
-    FEM_HIP(hipMemcpyAsync(lr, h.d_last_res, sizeof(lr), hipMemcpyDeviceToHost, b->stream));
-    FEM_HIP(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->insts.size(); i++) {
+    const InstHost& h = b->insts[i];
+    if (!h.d_counters) continue;
+    const char* rec = b->h_info + i * kInfoBytes;
+    const long long* cnt = (const long long*)rec;
+    const int* st = (const int*)(rec + kInfoStatus);
+    const double* lr = (const double*)(rec + kInfoLastRes);
     its += cnt[0];
     sw += cnt[1];
     steps = std::max(steps, cnt[2]);
@@ -864,6 +1015,9 @@ int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
   return FEM_OK;
 }
 
+// Abort flag of the last launch, from the host copy read_info made.
+int info_abort(const fem_batch* b) { return *(const int*)(b->h_info + b->insts.size() * kInfoBytes); }
+
 int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats) {
   if (inst < 0 || inst >= (int)b->insts.size()) {
     set_error("instance index out of range");
@@ -874,13 +1028,12 @@ int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats) {
     set_error("instance has no state yet (set_equation first)");
     return FEM_ERR_STATE;
   }
-  long long cnt[4];
-  int st[4];
-  double lr[kMaxK];
-  FEM_HIP(hipMemcpyAsync(cnt, h.d_counters, sizeof(cnt), hipMemcpyDeviceToHost, b->stream));
-  FEM_HIP(hipMemcpyAsync(st, h.d_status, sizeof(st), hipMemcpyDeviceToHost, b->stream));
-  FEM_HIP(hipMemcpyAsync(lr, h.d_last_res, sizeof(lr), hipMemcpyDeviceToHost, b->stream));
-  FEM_HIP(hipStreamSynchronize(b->stream));
+  int rc = read_info(b);
+  if (rc) return rc;
+  const char* rec = b->h_info + (size_t)inst * kInfoBytes;
+  const long long* cnt = (const long long*)rec;
+  const int* st = (const int*)(rec + kInfoStatus);
+  const double* lr = (const double*)(rec + kInfoLastRes);
   double res = 0.0;
   for (int j = 0; j < h.k; j++) res = std::max(res, lr[j]);
   stats->iterations = cnt[0];
@@ -889,6 +1042,19 @@ int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats) {
   stats->residual = res;
   stats->ms = 0.0;  // device time is per call, not per instance
   stats->status = st[0];
+  return 0;
+}
+
+int batch_plan_info(fem_batch* b, fem_plan_info* info) {
+  int rc = batch_prepare(b, true);
+  if (rc) return rc;
+  int maxb = 0;
+  rc = step_max_coresident(b->launch.k, b->launch.inner_fp32, b->launch.kr, b->launch.coh, &maxb);
+  if (rc) return rc;
+  info->grid = b->grid;
+  info->teams = b->nteams;
+  info->exchange = b->launch.coh;
+  info->coresident = maxb;
   return 0;
 }
 
@@ -915,10 +1081,9 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   FEM_HIP(hipEventSynchronize(b->ev1));
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  int abort_flag = 0;
-  FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
   rc = collect_stats(b, stats, ms);
-  if (abort_flag) {
+  if (rc == FEM_ERR_HIP || rc == FEM_ERR_OOM) return rc;
+  if (info_abort(b)) {
     for (auto& h : b->insts) h.hist = 0;
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
@@ -1004,10 +1169,9 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
   FEM_HIP(hipEventSynchronize(b->ev1));
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  int abort_flag = 0;
-  FEM_HIP(hipMemcpy(&abort_flag, b->d_abort, sizeof(int), hipMemcpyDeviceToHost));
   rc = collect_stats(b, stats, ms);
-  if (abort_flag) {
+  if (rc == FEM_ERR_HIP || rc == FEM_ERR_OOM) return rc;
+  if (info_abort(b)) {
     for (auto& h : b->insts) h.hist = 0;
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;

@@ -70,11 +70,19 @@ struct InstHost {
   double* d_x = nullptr;
   double* d_b = nullptr;
   void* d_vec = nullptr;     // slab of inner-precision vectors
-  int* d_status = nullptr;
+  int* d_status = nullptr;         // these three point into the batch's info block
   long long* d_counters = nullptr;
   double* d_last_res = nullptr;
+  float* d_surf = nullptr;   // surface-space staging of set_state / get_state [nv][k]
   Inst dev{};                // mirror of the device descriptor
 };
+
+// Per-instance outcome record in the batch's info block: counters [0, 32),
+// status bits [32, 48), final residuals [48, 48 + 8 kMaxK).  After the
+// instance records: the launch abort flag and the surface-change flag.
+constexpr size_t kInfoBytes = 256;
+constexpr size_t kInfoStatus = 32, kInfoLastRes = 48;
+static_assert(kInfoLastRes + 8 * kMaxK <= kInfoBytes, "info record");
 
 }  // namespace femhip
 
@@ -94,7 +102,12 @@ struct fem_batch {
   femhip::Slot* d_slots = nullptr;
   double* d_scal = nullptr;
   int grid = 0;
-  int* d_abort = nullptr;
+  int* d_abort = nullptr;       // in the info block
+  int* d_changed = nullptr;     // in the info block: set_state wrote a new value
+  char* d_info = nullptr;       // [ninst][kInfoBytes] + abort flag + change flag
+  char* h_info = nullptr;       // pinned host copy
+  size_t info_bytes = 0;
+  std::vector<femhip::Inst> uploaded;  // descriptors last copied to the plan's d_insts
   int nteams = 0;
   int grid_budget = 0;          // multi-instance plans: workgroups to spread over (0 = all co-resident)
   int mode_override = 0;        // exchange mode forced by the autotuner (0 = by the row-cache rule)
@@ -130,11 +143,14 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
 int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x);
 int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_clear(fem_batch* b, int inst);
+int batch_set_surface(fem_batch* b, int inst, int field, const float* values_nv);
+int batch_get_surface(fem_batch* b, int inst, int field, float* values_nv);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
 int batch_graph_capture(fem_batch* b, int nsteps);
 int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats);
+int batch_plan_info(fem_batch* b, fem_plan_info* info);
 int batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_surface_values(fem_batch* b, int inst, float* d_out);

@@ -149,6 +149,15 @@ class fem_ctx_info(C.Structure):
     ]
 
 
+class fem_plan_info(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32),
+        ("teams", C.c_int32),
+        ("exchange", C.c_int32),
+        ("coresident", C.c_int32),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 PD = C.POINTER(C.c_double)
@@ -194,6 +203,7 @@ SIGNATURES = [
     ("fem_batch_instance_stats", C.c_int, [P, I32, C.POINTER(fem_step_stats)]),
     ("fem_batch_graph_capture", C.c_int, [P, I32]),
     ("fem_batch_graph_launch", C.c_int, [P, C.POINTER(fem_step_stats)]),
+    ("fem_batch_plan_info", C.c_int, [P, C.POINTER(fem_plan_info)]),
     ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_operator_create", C.c_int, [P, PD, I32, I32, C.POINTER(P)]),
     ("fem_operator_destroy", C.c_int, [P]),

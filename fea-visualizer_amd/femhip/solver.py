@@ -10,13 +10,49 @@ only moves arrays across the boundary.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import itertools
+import sys
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
 
 from . import _lib as L
+
+# Every live handle, keyed by creation order.  At interpreter exit they are
+# closed explicitly, newest first (solvers before the contexts they use),
+# while the HIP runtime is still fully up; finalizers that run later during
+# interpreter teardown make no library call.
+_LIVE: "weakref.WeakValueDictionary[int, object]" = weakref.WeakValueDictionary()
+_ORDER = itertools.count()
+
+
+def _register(obj):
+    _LIVE[next(_ORDER)] = obj
+
+
+@atexit.register
+def close_all():
+    """Close every live handle, newest first."""
+    for key in sorted(list(_LIVE.keys()), reverse=True):
+        obj = _LIVE.get(key)
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+def _finalize(obj):
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
 
 __all__ = [
     "HeatParameters", "AdvectionDiffusionParameters", "WaveParameters",
@@ -141,6 +177,7 @@ class FEMContext:
         h = C.c_void_p()
         L.check(self._lib.fem_ctx_create(C.byref(m), int(boundary_condition), C.byref(h)))
         self.handle = h
+        _register(self)
         self.assemble_matrices(velocity, velocity_e, normalize_velocity, kappa_e)
 
     def assemble_matrices(self, velocity=None, velocity_e=None, normalize_velocity=True, kappa_e=None):
@@ -227,10 +264,7 @@ class FEMContext:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class HIPSolver:
@@ -244,6 +278,7 @@ class HIPSolver:
         o = self.options.to_c()
         L.check(self._lib.fem_solver_create(fem_ctx.handle, C.byref(o), C.byref(h)))
         self.handle = h
+        _register(self)
         self.last_stats = L.fem_step_stats()
         self._eq = None
         self.set_equation()
@@ -349,10 +384,7 @@ class HIPSolver:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Batch:
@@ -367,6 +399,7 @@ class Batch:
         o = self.options.to_c()
         L.check(self._lib.fem_batch_create(arr, len(self.contexts), C.byref(o), C.byref(h)))
         self.handle = h
+        _register(self)
         self.last_stats = L.fem_step_stats()
 
     def set_equation(self, instance: int, params):
@@ -405,6 +438,13 @@ class Batch:
         """Replay the recorded advance; returns the step status as advance()."""
         return L.check(self._lib.fem_batch_graph_launch(self.handle, C.byref(self.last_stats)))
 
+    def plan_info(self):
+        """Persistent-engine launch plan: grid, teams, exchange mode and the
+        co-resident limit of the launched variant (fem_plan_info)."""
+        info = L.fem_plan_info()
+        L.check(self._lib.fem_batch_plan_info(self.handle, C.byref(info)))
+        return info
+
     def autotune(self, probe_steps: int = 5):
         """Pick the workgroup budget of the team plan (state restored);
         returns (budget, best probe ms)."""
@@ -421,10 +461,7 @@ class Batch:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Operator:
@@ -443,6 +480,7 @@ class Operator:
         L.check(self._lib.fem_operator_create(fem_ctx.handle, L.ptr(c, C.c_double), int(precision),
                                               1 if transpose else 0, C.byref(h)))
         self.handle = h
+        _register(self)
 
     def apply(self, x):
         """y = S x for x of shape [N] or [N, nrhs] (host arrays, FP64 in and out)."""
@@ -483,10 +521,7 @@ class Operator:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Adjoint:
@@ -502,6 +537,7 @@ class Adjoint:
         o = self.options.to_c()
         L.check(self._lib.fem_adjoint_create(fem_ctx.handle, C.byref(o), C.byref(h)))
         self.handle = h
+        _register(self)
         self.forward_stats = L.fem_step_stats()
         self.backward_stats = L.fem_step_stats()
         self.set_equation(params)
@@ -535,7 +571,4 @@ class Adjoint:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)

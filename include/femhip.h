@@ -216,8 +216,13 @@ int fem_ctx_smooth_normals(fem_ctx* ctx, const float* normals_nv3, const float* 
  * fem_solver_clear    <- Solver::clear_values        (CPUSolver.cpp:54-59)
  * fem_solver_unstable <- Solver::has_numerical_instability (CPUSolver.cpp:41-48)
  * set/get_state work in surface space (nv values, gather/scatter through
- * idx_map as CPUSolver.cpp:13-37; boundary values read back as 0).  With
- * nrhs > 1 the arrays hold nv*nrhs values, right-hand side fastest. */
+ * idx_map as CPUSolver.cpp:13-37, on the device; boundary values read back
+ * as 0).  set_state keeps every unknown whose FP64 state, rounded to float,
+ * equals the value given, so writing back what get_state returned (the
+ * reference's gather/step/scatter frame loop) leaves the FP64 state and the
+ * predictor history as they were; any other value replaces the state and
+ * restarts the predictor.  With nrhs > 1 the arrays hold nv*nrhs values,
+ * right-hand side fastest. */
 int fem_solver_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_solver** out);
 int fem_solver_destroy(fem_solver* s);
 int fem_solver_set_equation(fem_solver* s, int32_t equation, const void* params, size_t size);
@@ -269,6 +274,19 @@ int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* sta
  * and GL-compatible engines are host-driven and are not captured. */
 int fem_batch_graph_capture(fem_batch* b, int32_t nsteps);
 int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats);
+/* Launch plan of the persistent engine (built here if it is not yet):
+ * workgroups in the grid, teams, the exchange mode of the launched variant
+ * (0: one-workgroup teams, FEM_EXCHANGE_COHERENT or FEM_EXCHANGE_FENCED) and
+ * how many workgroups of that variant the device holds at once.  The
+ * stepping kernel's barriers need grid <= coresident; every launch checks
+ * it.  No reference counterpart. */
+typedef struct fem_plan_info {
+  int32_t grid;
+  int32_t teams;
+  int32_t exchange;
+  int32_t coresident;
+} fem_plan_info;
+int fem_batch_plan_info(fem_batch* b, fem_plan_info* info);
 /* Autotuner over the number of workgroups the batch plan spreads over
  * (*chosen; 0 when the streaming engine won); state restored as for
  * fem_solver_autotune. */

@@ -4,12 +4,18 @@
 // instability rule (clearing and stopping on it, as the reference pauses),
 // and scatters the values back.
 //
-// usage: solver_driver <in.bin> <out.bin> <equation> <steps> <p0> <p1> [p2 p3 p4]
+// usage: solver_driver <in.bin> <out.bin> <equation> <steps> <p0> <p1> [p2 p3 p4 [p5]]
+//                      [--velocity-at FRAME X Y Z]
 //   in.bin : int32 nv, int32 ntri, int32 bc, float xyz[3 nv], uint32 tri[3 ntri],
 //            uint8 on_boundary[nv], float values[nv]
 //   out.bin: int32 frames_run, int32 unstable, float values[nv]
 //   equation: heat | advdiff | wave | grayscott  (parameters in the order of
-//             EquationParameters.hpp; advdiff uses the default velocity)
+//             EquationParameters.hpp; grayscott's p5 is the reaction
+//             weighting, 0 nodal as the reference, 1 lumped; advdiff starts
+//             with the default velocity)
+//   --velocity-at: before frame FRAME the velocity changes to (X, Y, Z) and
+//             the matrices are re-assembled, as the reference does when its
+//             velocity slider moves (Application.cpp:394)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -47,8 +53,19 @@ int main(int argc, char** argv) {
 
     const std::string eq = argv[3];
     const int steps = atoi(argv[4]);
-    double p[5] = {0, 0, 0, 0, 0};
-    for (int i = 5; i < argc && i < 10; i++) p[i - 5] = atof(argv[i]);
+    double p[6] = {0, 0, 0, 0, 0, 0};
+    int np = 0, switch_frame = -1;
+    double vel2[3] = {0, 0, 0};
+    for (int i = 5; i < argc; i++) {
+      if (std::string(argv[i]) == "--velocity-at") {
+        if (i + 4 >= argc) throw std::runtime_error("--velocity-at needs FRAME X Y Z");
+        switch_frame = atoi(argv[i + 1]);
+        for (int q = 0; q < 3; q++) vel2[q] = atof(argv[i + 2 + q]);
+        i += 4;
+      } else if (np < 6) {
+        p[np++] = atof(argv[i]);
+      }
+    }
 
     femhip::Context ctx(xyz.data(), nv, tri.data(), nt, onb.data(), bc);
     femhip::HIPSolver solver(ctx);
@@ -69,6 +86,7 @@ int main(int argc, char** argv) {
       r.Dv = p[2];
       r.feed_rate = p[3];
       r.kill_rate = p[4];
+      r.reaction_weighting = (int)p[5];
       solver.set_equation(r);
     } else {
       throw std::runtime_error("unknown equation " + eq);
@@ -76,6 +94,7 @@ int main(int argc, char** argv) {
     solver.attach_surface_values(values.data());
     int frames = 0, unstable = 0;
     for (; frames < steps; frames++) {
+      if (frames == switch_frame) ctx.assemble(vel2);  // FEMContext::assemble_matrices
       solver.advance_time();
       if (solver.has_numerical_instability()) {
         // Application::clear_solver (Application.cpp:579-583): solver and surface values

@@ -7,9 +7,11 @@ by scipy COO summation, and every time step solved with scipy's sparse direct
 solver.  The oracle restates the reference's J^{-T} construction and Eigen's
 CG instead, so agreement between the two pins both.
 
-Meshes: the 64x64 structured unit square of the heat benchmark and the
+Meshes: the 64x64 structured unit square of the heat benchmark, the
 reference's shipped assets/fem_meshes/icosphere.obj (10,242 vertices), whose
-arrays are stored in the fixture so the tests do not need the reference tree.
+arrays are stored in the fixture so the tests do not need the reference tree,
+and a jittered 24x24 lattice with per-element diffusivity and per-element,
+unnormalised velocity.
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -26,12 +28,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "fea-visualizer_amd"))
 
-from femhip.meshes import read_obj, structured_square  # noqa: E402
+from femhip.meshes import jittered_square, read_obj, structured_square  # noqa: E402
 
 REF_ICO = "/root/reference/assets/fem_meshes/icosphere.obj"
 
 
-def element_data(xyz, tri, velocity, normalize=True):
+def element_data(xyz, tri, velocity, normalize=True, velocity_e=None):
     X = xyz.astype(np.float64)
     a, b, c = X[tri[:, 0]], X[tri[:, 1]], X[tri[:, 2]]
     n = np.cross(b - a, c - a)
@@ -40,21 +42,25 @@ def element_data(xyz, tri, velocity, normalize=True):
     nh = n / ln[:, None]
     edges = [c - b, a - c, b - a]  # opposite vertex 0, 1, 2
     grads = np.stack([np.cross(nh, e) / (2 * area[:, None]) for e in edges], axis=1)  # [T,3,3]
-    v = np.asarray(velocity, dtype=np.float64)[None, :]
+    if velocity_e is None:
+        v = np.asarray(velocity, dtype=np.float64)[None, :]
+    else:
+        v = np.asarray(velocity_e, dtype=np.float64).reshape(-1, 3)
     vt = v - (v * nh).sum(1, keepdims=True) * nh
     if normalize:
         vt = vt / np.linalg.norm(vt, axis=1, keepdims=True)
     return area, grads, vt
 
 
-def assemble(mesh, bc, velocity=(1.0, 0.0, 0.0)):
+def assemble(mesh, bc, velocity=(1.0, 0.0, 0.0), normalize=True, velocity_e=None, kappa_e=None):
     xyz, tri, onb = mesh["xyz"], mesh["tri"].astype(np.int64), mesh["on_boundary"].astype(bool)
     nv = xyz.shape[0]
     idx = -np.ones(nv, dtype=np.int64)
     keep = np.ones(nv, bool) if bc == 1 else ~onb
     idx[keep] = np.arange(keep.sum())
     n = int(keep.sum())
-    area, g, vt = element_data(xyz, tri, velocity)
+    area, g, vt = element_data(xyz, tri, velocity, normalize, velocity_e)
+    kap = np.ones(tri.shape[0]) if kappa_e is None else np.asarray(kappa_e, dtype=np.float64)
     rows, cols, kv, mv, av, pv = [], [], [], [], [], []
     for i in range(3):
         for j in range(3):
@@ -62,7 +68,7 @@ def assemble(mesh, bc, velocity=(1.0, 0.0, 0.0)):
             keep_km = np.ones_like(bi) if bc == 1 else (~bi & ~bj)
             keep_a = ~bi & ~bj
             r, cc = idx[tri[:, i]], idx[tri[:, j]]
-            kij = area * (g[:, i] * g[:, j]).sum(1)
+            kij = kap * area * (g[:, i] * g[:, j]).sum(1)
             mij = area / 12.0 * (2.0 if i == j else 1.0)
             aij = area / 3.0 * (vt * g[:, j]).sum(1)
             rows.append(r[keep_km]); cols.append(cc[keep_km])
@@ -138,6 +144,16 @@ def gs_steps(mats, u, v, dt, Du, Dv, F, k, nsteps, lumped):
     return np.array(us), np.array(vs)
 
 
+def patches(xyz, centres, radius):
+    """1 inside any geodesic disc of `radius` (radians) around the unit
+    `centres` on the sphere through the mesh's vertices, else 0."""
+    X = xyz.astype(np.float64)
+    X = X / np.linalg.norm(X, axis=1, keepdims=True)
+    c = np.asarray(centres, dtype=np.float64)
+    c = c / np.linalg.norm(c, axis=1, keepdims=True)
+    return (np.arccos(np.clip(X @ c.T, -1.0, 1.0)) < radius).any(1).astype(np.float64)
+
+
 def main():
     # ---- heat benchmark mesh: 64x64 structured square, Dirichlet
     sq = structured_square(64)
@@ -162,8 +178,29 @@ def main():
             u0 = np.zeros(n)  # CPUSolver::clear_values leaves u = 0
             gu, gv = gs_steps(mats, u0, v0, 1e-3, 0.08, 0.04, 0.035, 0.06, 3, lumped=False)
             hu = heat_steps(mats, v0, 0.01, 0.05, 3)
+            # the benchmark's Gray-Scott regime: dt = 0.5, u = 1, v = 0.25 on
+            # six patches, reaction weighted by the lumped mass (row sums of M)
+            cen = [(0, 0, 1), (0, 0, -1), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0.6, 0.6, 0.5)]
+            lv0 = 0.25 * patches(ico["xyz"], cen, 0.15)
+            lu0 = np.ones(n)
+            lu, lv = gs_steps(mats, lu0, lv0, 0.5, 2e-4, 1e-4, 0.037, 0.06, 3, lumped=True)
             np.savez_compressed(os.path.join(HERE, "icosphere_shipped.npz"), xyz=ico["xyz"], tri=ico["tri"],
-                                on_boundary=ico["on_boundary"], v0=v0, gs_u=gu, gs_v=gv, heat=hu, **asm)
+                                on_boundary=ico["on_boundary"], v0=v0, gs_u=gu, gs_v=gv, heat=hu,
+                                lumped_v0=lv0, lumped_u=lu, lumped_v=lv, **asm)
+    # ---- jittered lattice with per-element diffusivity and per-element,
+    # unnormalised velocity (the differentiable workload's inputs), Dirichlet
+    jm = jittered_square(24, seed=11)
+    cen = jm["xyz"].astype(np.float64)[jm["tri"].astype(np.int64)].mean(axis=1)
+    ve = np.stack([-(cen[:, 1] - 0.5), cen[:, 0] - 0.5, np.zeros(len(cen))], axis=1)
+    rk = np.random.Generator(np.random.PCG64(17))
+    kap = 1e-3 * np.exp(0.3 * rk.standard_normal(jm["tri"].shape[0]))
+    asm, mats, n = assemble(jm, 0, normalize=False, velocity_e=ve, kappa_e=kap)
+    X = jm["xyz"].astype(np.float64)
+    keep = asm["idx_map"] >= 0
+    u0 = np.exp(-((X[:, 0] - 0.3) ** 2 + (X[:, 1] - 0.5) ** 2) / (2 * 0.05 ** 2))[keep]
+    adv = advdiff_steps(mats, u0, 2e-3, 1.0, 3)
+    np.savez_compressed(os.path.join(HERE, "jittered_kappa.npz"), xyz=jm["xyz"], tri=jm["tri"],
+                        on_boundary=jm["on_boundary"], velocity_e=ve, kappa_e=kap, u0=u0, advdiff=adv, **asm)
     print("fixtures written to", HERE)
 
 

@@ -54,17 +54,22 @@ def run_driver(tmp_path, mesh, bc, values, equation, steps, *params):
         pytest.fail(f"{DRIVER} is missing: run __graft_entry__.build() first")
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     write_input(fin, mesh, bc, values)
-    res = subprocess.run([DRIVER, str(fin), str(fout), equation, str(steps)] + [repr(float(p)) for p in params],
+    args = [p if isinstance(p, str) else repr(float(p)) for p in params]
+    res = subprocess.run([DRIVER, str(fin), str(fout), equation, str(steps)] + args,
                          capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     return read_output(fout, mesh["xyz"].shape[0])
 
 
-def oracle_frames(s, equation, prm, steps, field0, state):
-    """The reference loop with float32 surface values between frames."""
+def oracle_frames(s, equation, prm, steps, field0, state, switch=None):
+    """The reference loop with float32 surface values between frames;
+    switch = (frame, system): from that frame on, step with the other
+    system (re-assembled matrices)."""
     u, v = state
     surf = field0.astype(np.float32)
-    for _ in range(steps):
+    for f in range(steps):
+        if switch is not None and f == switch[0]:
+            s = switch[1]
         x = s.gather(surf.astype(np.float64))
         if equation == oracle.GRAYSCOTT:
             v = x
@@ -103,3 +108,41 @@ def test_driver_grayscott_nodal_instability_clears(gpu, tmp_path):
     frames, unstable, got = run_driver(tmp_path, m, 1, vals, "grayscott", 20, 0.5, 2e-4, 1e-4, 0.037, 0.06)
     assert unstable == 1 and frames < 20
     assert not np.any(got)
+
+
+@pytest.mark.gpu
+def test_driver_advdiff_velocity_change(gpu, tmp_path):
+    """Advection-diffusion through the adapter (BiCGSTAB), with the velocity
+    changed and the matrices re-assembled after 5 of 10 frames, against the
+    oracle stepping with the matching matrices."""
+    m = meshes.jittered_square(24, seed=8)
+    x = m["xyz"].astype(np.float64)
+    vals = np.exp(-((x[:, 0] - 0.4) ** 2 + (x[:, 1] - 0.5) ** 2) / 0.02)
+    v2 = (-0.3, 0.9, 0.0)
+    prm = [2e-3, 0.05]
+    print("case driver advdiff velocity change", flush=True)
+    frames, unstable, got = run_driver(tmp_path, m, 0, vals, "advdiff", 10, *prm, "--velocity-at", 5, *v2)
+    assert frames == 10 and unstable == 0
+    s1 = oracle.System(m, bc=0, velocity=(1.0, 0.0, 0.0))
+    s2 = oracle.System(m, bc=0, velocity=v2)
+    ref = oracle_frames(s1, oracle.ADVDIFF, prm, 10, vals, (np.zeros(s1.n), np.zeros(s1.n)), switch=(5, s2))
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    # without the re-assembly the result differs
+    stale = oracle_frames(s1, oracle.ADVDIFF, prm, 10, vals, (np.zeros(s1.n), np.zeros(s1.n)))
+    assert np.linalg.norm(got - stale) > 1e-3 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+def test_driver_grayscott_lumped_stable(gpu, tmp_path):
+    """Gray-Scott with the lumped weighting at dt = 0.5 stays stable through
+    the adapter; the surface carries v (CPUSolver.cpp:130,144) and u stays on
+    the device, starting from 0 as after clear_values (CPUSolver.cpp:54-59)."""
+    m = meshes.icosphere(3)
+    s = oracle.System(m, bc=1)
+    vals = np.where(np.arange(m["xyz"].shape[0]) % 7 == 0, 0.25, 0.0)
+    prm = [0.5, 2e-4, 1e-4, 0.037, 0.06, 1]
+    print("case driver grayscott lumped", flush=True)
+    frames, unstable, got = run_driver(tmp_path, m, 1, vals, "grayscott", 12, *prm)
+    assert frames == 12 and unstable == 0
+    ref = oracle_frames(s, oracle.GRAYSCOTT, prm, 12, vals, (np.zeros(s.n), np.zeros(s.n)))
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)

@@ -111,7 +111,9 @@ def test_golden_icosphere_shipped():
     assert int(m["on_boundary"].sum()) == 0
     assert np.array_equal(s.row_ptr, g["row_ptr"])
     assert np.array_equal(s.col_idx, g["col_idx"])
-    for name in "KM":
+    # K, M and the curved-surface advection A (velocity (1,0,0) projected on
+    # each element's plane and normalised)
+    for name in "KMA":
         assert np.max(np.abs(getattr(s, name) - g[name])) <= 1e-12 * np.max(np.abs(g[name]))
     # total area of the shipped icosphere (sum of all consistent-mass entries)
     assert abs(s.M.sum() - 12.5626) < 1e-3
@@ -122,6 +124,46 @@ def test_golden_icosphere_shipped():
         _, u, v, _ = s.step(oracle.GRAYSCOTT, prm, u, v, tol=1e-14)
         assert np.linalg.norm(u - g["gs_u"][k]) / np.linalg.norm(g["gs_u"][k]) < 1e-11
         assert np.linalg.norm(v - g["gs_v"][k]) / np.linalg.norm(g["gs_v"][k]) < 1e-11
+
+
+def test_golden_icosphere_lumped_grayscott():
+    """The benchmark's Gray-Scott regime (dt = 0.5, u = 1, patches of v = 0.25)
+    with the reaction weighted by the lumped mass, three steps, against the
+    direct-solve fixture."""
+    g = load_gold("icosphere_shipped.npz")
+    s = oracle.System(mesh_of(g), bc=1)
+    u = np.ones(s.n)
+    v = g["lumped_v0"].copy()
+    prm = [0.5, 2e-4, 1e-4, 0.037, 0.06, 1]
+    for k in range(3):
+        _, u, v, _ = s.step(oracle.GRAYSCOTT, prm, u, v, tol=1e-14)
+        assert np.linalg.norm(u - g["lumped_u"][k]) / np.linalg.norm(g["lumped_u"][k]) < 1e-11
+        assert np.linalg.norm(v - g["lumped_v"][k]) / np.linalg.norm(g["lumped_v"][k]) < 1e-11
+    # the lumped weighting changes the answer: the nodal one differs at step 1
+    _, un, vn, _ = s.step(oracle.GRAYSCOTT, [0.5, 2e-4, 1e-4, 0.037, 0.06, 0], np.ones(s.n),
+                          g["lumped_v0"].copy(), tol=1e-14)
+    assert np.linalg.norm(un - g["lumped_u"][0]) / np.linalg.norm(g["lumped_u"][0]) > 1e-3
+
+
+def test_golden_jittered_kappa_velocity():
+    """Per-element diffusivity and per-element, unnormalised velocity: K, M,
+    A and three backward-Euler advection-diffusion steps against the fixture
+    built with the barycentric-gradient formula and a direct solver."""
+    g = load_gold("jittered_kappa.npz")
+    s = oracle.System(mesh_of(g), bc=0, velocity_e=g["velocity_e"], normalize=False, kappa_e=g["kappa_e"])
+    assert np.array_equal(s.idx_map, g["idx_map"])
+    assert np.array_equal(s.row_ptr, g["row_ptr"]) and np.array_equal(s.col_idx, g["col_idx"])
+    for name in "KMA":
+        assert np.max(np.abs(getattr(s, name) - g[name])) <= 1e-12 * np.max(np.abs(g[name]))
+    u = g["u0"].copy()
+    for k in range(3):
+        _, u, _, _ = s.step(oracle.ADVDIFF, [2e-3, 1.0], u, krylov=oracle.BICGSTAB, tol=1e-14)
+        assert np.linalg.norm(u - g["advdiff"][k]) / np.linalg.norm(g["advdiff"][k]) < 1e-11
+    # a dropped kappa or a normalised velocity would not match
+    s1 = oracle.System(mesh_of(g), bc=0, velocity_e=g["velocity_e"], normalize=False)
+    assert np.max(np.abs(s1.K - g["K"])) > 1e-3 * np.max(np.abs(g["K"]))
+    s2 = oracle.System(mesh_of(g), bc=0, velocity_e=g["velocity_e"], normalize=True, kappa_e=g["kappa_e"])
+    assert np.max(np.abs(s2.A - g["A"])) > 1e-3 * np.max(np.abs(g["A"]))
 
 
 def cot(a, b, c):
