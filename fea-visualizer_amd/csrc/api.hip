@@ -432,6 +432,17 @@ int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opt
   })
 }
 
+int fem_ctx_solve_noalloc(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b,
+                          double* x, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!ctx || !coef || (ctx->n > 0 && (!b || !x))) {
+      set_error("fem_ctx_solve_noalloc: null argument");
+      return FEM_ERR_INVALID;
+    }
+    return linear_solve_noalloc(ctx, coef, opts, b, x, stats);
+  })
+}
+
 int fem_ctx_smooth_normals(fem_ctx* ctx, const float* normals_nv3, const float* values_nv, float extrusion,
                            float* out_nv3) {
   FEM_GUARD({

@@ -301,6 +301,64 @@ void free_plan(fem_batch* b) {
   b->plan_ready = false;
 }
 
+}  // namespace
+
+// Workgroups of the team of a lone instance, within maxb co-resident
+// workgroups: team_size when given and it fits, else about one row per lane
+// (at most 1024).
+int single_team_size(long long n, int k, int team_size, int maxb) {
+  if (team_size > 0 && team_size <= maxb) return team_size;
+  const long long lanes = n * k;
+  const int g = (int)std::max(1LL, (lanes + kBlock - 1) / kBlock);
+  return std::min(g, std::min(maxb, 1024));
+}
+
+// A workgroup of a multi-workgroup team whose rows exceed two per lane (the
+// kernel's LDS row cache) streams its phases.
+bool team_streams(long long n, int k, int size) {
+  const long long rows = (n + size - 1) / size;
+  return rows * k > 2LL * kBlock;
+}
+
+// Exchange mode of a plan: plain for one-workgroup teams only; else forced,
+// then the autotuner's choice, then fenced for streaming teams, coherent
+// otherwise.
+int exchange_mode(bool multi, bool streaming, int forced, int tuned) {
+  return !multi ? 0 : forced ? forced : tuned ? tuned : streaming ? kExchangeFenced : kExchangeCoherent;
+}
+
+// S = coef[0] M + coef[1] K + coef[2] A (one value column) and its Jacobi
+// inverse diagonal, FP64 and inner precision, into caller-given arrays.
+int form_system(fem_ctx* c, const double coef[3], int fp32, double* S64, void* Si, double* dinv64, void* dinvi,
+                hipStream_t st) {
+  if (c->n == 0) return 0;
+  Coef cf;
+  std::memset(&cf, 0, sizeof(cf));
+  cf.cM[0] = coef[0];
+  cf.cK[0] = coef[1];
+  cf.cA[0] = coef[2];
+  if (fp32)
+    hipLaunchKernelGGL(k_form<float>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, st, c->n, c->d_rp, c->d_diag,
+                       nullptr, c->d_M, c->d_K, c->d_A, 1, cf, S64, (float*)Si, dinv64, (float*)dinvi);
+  else
+    hipLaunchKernelGGL(k_form<double>, dim3(div_up(c->n, kBlock)), dim3(kBlock), 0, st, c->n, c->d_rp, c->d_diag,
+                       nullptr, c->d_M, c->d_K, c->d_A, 1, cf, S64, (double*)Si, dinv64, (double*)dinvi);
+  FEM_HIP(hipGetLastError());
+  return 0;
+}
+
+// Barrier slots, team sums, abort flag and the instances' outcome records
+// reset before a launch of the stepping kernel (a fill kernel).
+int launch_slot_reset(Slot* slots, int nslots, double* scal, int nteams, int* abort_flag, Inst* insts, int ninst,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_reset, dim3(div_up(std::max(nteams + nslots, ninst), kBlock)), dim3(kBlock), 0, st, slots,
+                     nslots, scal, nteams, abort_flag, insts, ninst);
+  FEM_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
 // Team plan.  Work of an instance ~ (nnz + n) * k per Krylov iteration.  With
 // one instance the team takes ~4 rows per lane, capped by the co-resident
 // grid.  With many, instances heavier than the per-workgroup share get teams
@@ -344,17 +402,14 @@ int build_plan(fem_batch* b) {
   auto make_plan = [&](int maxb) {
     teams.clear();
     sizes.clear();
-    if (b->opts.team_size > 0 && (long long)b->opts.team_size * ninst <= maxb) {
+    if (ninst == 1) {
+      teams.push_back({0});
+      sizes.push_back(single_team_size(b->insts[0].ctx->n, k, b->opts.team_size, maxb));
+    } else if (b->opts.team_size > 0 && (long long)b->opts.team_size * ninst <= maxb) {
       for (int i = 0; i < ninst; i++) {
         teams.push_back({i});
         sizes.push_back(b->opts.team_size);
       }
-    } else if (ninst == 1) {
-      const long long lanes = (long long)b->insts[0].ctx->n * k;
-      int g = (int)std::max(1LL, (lanes + kBlock - 1) / kBlock);  // ~one row per lane
-      g = std::min(g, std::min(maxb, 1024));
-      teams.push_back({0});
-      sizes.push_back(g);
     } else {
       const int budget = (b->grid_budget > 0) ? std::min(b->grid_budget, maxb) : maxb;
       const double share = W / budget;
@@ -404,15 +459,10 @@ int build_plan(fem_batch* b) {
     for (size_t t = 0; t < teams.size(); t++) {
       if (sizes[t] <= 1) continue;
       multi = true;
-      for (int i : teams[t]) {
-        const long long rows = (b->insts[i].ctx->n + sizes[t] - 1) / sizes[t];
-        if (rows * k > 2LL * kBlock) streaming = true;
-      }
+      for (int i : teams[t])
+        if (team_streams(b->insts[i].ctx->n, k, sizes[t])) streaming = true;
     }
-    return !multi ? 0
-           : forced ? forced
-           : b->mode_override ? b->mode_override
-           : streaming ? kExchangeFenced : kExchangeCoherent;
+    return exchange_mode(multi, streaming, forced, b->mode_override);
   };
   auto plan_grid = [&]() {
     int g = 0;

@@ -25,6 +25,13 @@ constexpr int kExchangeCoherent = 1, kExchangeFenced = 2;
 
 int launch_step(const StepLaunch& L, int nsteps, hipStream_t st);
 
+// Plan rules shared by the batch plans and the allocation-free solve (batch.hip).
+int single_team_size(long long n, int k, int team_size, int maxb);
+bool team_streams(long long n, int k, int size);
+int exchange_mode(bool multi, bool streaming, int forced, int tuned);
+int launch_slot_reset(Slot* slots, int nslots, double* scal, int nteams, int* abort_flag, Inst* insts, int ninst,
+                      hipStream_t st);
+
 // Streaming engine (stream.hip): row-block table and per-instance scalars.
 struct StreamPlan;
 struct InstHost;
@@ -158,4 +165,8 @@ int gl_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_set_linear(fem_batch* b, int inst, const double coef[3]);
 int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                  fem_step_stats* stats);
+int form_system(fem_ctx* c, const double coef[3], int fp32, double* S64, void* Si, double* dinv64, void* dinvi,
+                hipStream_t st);
+int linear_solve_noalloc(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
+                         fem_step_stats* stats);
 }  // namespace femhip

@@ -178,6 +178,7 @@ SIGNATURES = [
     ("fem_ctx_get_pattern", C.c_int, [P, PI, PI]),
     ("fem_ctx_get_matrix", C.c_int, [P, I32, PD]),
     ("fem_ctx_solve", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
+    ("fem_ctx_solve_noalloc", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
     ("fem_ctx_smooth_normals", C.c_int, [P, PF, PF, C.c_float, PF]),
     ("fem_solver_create", C.c_int, [P, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_solver_destroy", C.c_int, [P]),

@@ -200,6 +200,19 @@ int fem_ctx_get_matrix(const fem_ctx* ctx, int32_t which, double* values);
  * opts->warm_start, else x0 = 0 as Eigen's solve(). */
 int fem_ctx_solve(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                   fem_step_stats* stats);
+/* fem_ctx_solve without any allocation: no reference counterpart; for callers
+ * that cannot allocate device memory (tight serving loops, code that must not
+ * call hipMalloc).  Same arguments, semantics and results as fem_ctx_solve,
+ * on the persistent engine only.  Every device buffer the solve needs beyond
+ * the caller's b and x (formed matrix, Krylov vectors, device copies of b and
+ * x, barrier slots) lives in a static 32 MiB array per GPU that the library
+ * holds until the process ends; systems that need more return
+ * FEM_ERR_INVALID (about 180,000 unknowns fit at one right-hand side in
+ * FP32 + refinement).  Calls on one GPU share that array: calls that overlap
+ * on one GPU race.  Runs on the context's stream; stats->ms is the call's
+ * wall time. */
+int fem_ctx_solve_noalloc(fem_ctx* ctx, const double coef[3], const fem_solver_opts* opts, const double* b,
+                          double* x, fem_step_stats* stats);
 /* Smooth vertex normals of the surface extruded by per-vertex values
  * (shaders/FEM/smooth_normals.glsl:22-57): every vertex moves along its
  * normal by min(1, extrusion * max(0, value)); each triangle's unnormalised
