@@ -1,18 +1,27 @@
 """Benchmark: DOF·steps/s of the FEM time stepper on one MI355X.
 
-Default workload: Gray-Scott on an icosphere refined 6 times (40,962
-vertices, 81,920 triangles, radius 1), two species, Du=2e-4, Dv=1e-4,
-F=0.037, k=0.06, dt=0.5, u=1 and v=0 plus 50 seeded patches
-(v = 0.25 + U(0, 0.05)), implicit diffusion and explicit reaction,
-2,000 steps, FP32 CG with FP64 iterative refinement (tol 1e-10).  The
-reaction uses lumped-mass weighting: the reference's nodal weighting
-(CPUSolver.cpp:134,137) diverges at dt = 0.5 (u reaches -143 in one step).
+Default workload (the headline line): Gray-Scott on an icosphere refined 6
+times (40,962 vertices, 81,920 triangles, radius 1), two species, Du=2e-4,
+Dv=1e-4, F=0.037, k=0.06, dt=0.5, u=1 and v=0 plus 50 seeded patches
+(v = 0.25 + U(0, 0.05)), implicit diffusion and explicit reaction, FP32 CG
+with FP64 iterative refinement (tol 1e-10).  The reaction uses lumped-mass
+weighting: the reference's nodal weighting (CPUSolver.cpp:134,137) diverges
+at dt = 0.5 (u reaches -143 in one step).
 
 A step is one implicit time step of both species; DOF = N x 2 species.
-Prints one JSON line.  Extra keys: "roofline" (the persistent stepping
-kernel against the 8 TB/s HBM peak), "cpu_baseline" (the FP32 port of the
-reference CPU path, one core), "accuracy" (rel-L2 against FP64) and
-"plain_fp64_gpu" (the plain FP64 GPU CG engine on the same workload).
+`--warmup W` untimed steps run from the initial state, then `--steps K`
+timed steps continue the same trajectory in one launch (steps W+1..W+K).
+Prints one JSON line.  Extra keys:
+  roofline          the persistent stepping kernel against the 8 TB/s HBM peak
+  accuracy          one-step (teacher-forced) and trajectory rel-L2 against the
+                    oracle (C restatement of the reference CPU path in FP64)
+  fp64_same_engine  the same kernel and steps with FP64 Krylov vectors
+  plain_fp64_gpu    the plain FP64 engine (host-read reduction scalars)
+  frames            per-frame cost of the reference's call pattern through the
+                    surface-space Solver boundary (set_state, 1 step, get_state)
+  spmv, adjoint     the SpMV bandwidth and adjoint/forward cost legs
+  batched           the 256-mesh x 16-RHS batched line, K steps after W
+  cpu_baseline      the FP32 port of the reference CPU path, one core
 """
 from __future__ import annotations
 
@@ -32,11 +41,12 @@ for _p in (ROOT, os.path.join(ROOT, "fea-visualizer_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM-side bytes measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
-# against a plain copy kernel) for the dominant kernel of a workload; the
-# summaries with their commands are in profiles/.
+# Bytes of the dominant kernel's dispatch measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE (calibrated against a plain copy kernel) for the
+# exact command given; the summaries with their commands are in profiles/.
 PROFILED_TRAFFIC = {
-    "grayscott": {"bytes_per_step": 3.9685e7, "source": "profiles/k_step_grayscott_icosphere6.md"},
+    "grayscott": {"steps": 20, "warmup": 5, "bytes_per_launch": None,
+                  "source": "profiles/k_step_grayscott_icosphere6.md"},
     "spmv": {"bytes_per_launch": 1.2087e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
 }
 METRIC = "DOF·steps/s at rel-L2≤1e-8 vs FP64"
@@ -271,10 +281,10 @@ def run_heat_c1(args):
     wall = time.perf_counter() - t
     st = sol.last_stats
     ach = step_bytes(n, nnz, 1, 1, st.iterations, st.sweeps, steps, ti=8, fused=True) / (st.ms * 1e-3) / 1e9
-    ref = HIPSolver(ctx, SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-13))
-    ref.set_state_unknowns(_lib.FEM_FIELD_U, u0)
-    ref.advance(steps)
-    uu, ur = sol.get_state_unknowns(), ref.get_state_unknowns()
+    import oracle
+    so = oracle.System(mesh, bc=0)
+    _, ur, _, _ = so.run(oracle.HEAT, [1e-3, 1.0], steps, u0.copy(), tol=1e-14)
+    uu = sol.get_state_unknowns()
     out = contract_line(METRIC, n * steps / wall, "DOF·steps/s", steps, args.warmup, 1e3 * wall / steps, "f64",
                         {"workload": f"heat, structured 64x64 unit square (N={n}, nnz={nnz}), dt=1e-3, "
                                      f"conductivity 1, Dirichlet, Gaussian u0, FP64 CG tol 1e-10",
@@ -289,9 +299,8 @@ def run_heat_c1(args):
                                   "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step<1,double>",
                                   "note": "1.6 MB working set: cache-resident, latency-bound"},
                         accuracy={"trajectory_rel_l2_final": float(np.linalg.norm(uu - ur) / np.linalg.norm(ur)),
-                                  "reference": "FP64 GPU CG, tol 1e-13"})
+                                  "reference": "oracle (C restatement, FP64 CG to 1e-14), same steps"})
     if not args.no_cpu:
-        import oracle
         s = oracle.System(mesh, bc=0, dtype=np.float32)
         prep = oracle.port_prepare(s)
         t = time.perf_counter()
@@ -348,6 +357,15 @@ def run_wave_c3(args):
     ref = fresh(precision=_lib.FEM_PREC_FP64, tol=1e-13)
     ref.advance(steps)
     uu, ur = sol.get_state_unknowns(), ref.get_state_unknowns()
+    # one step teacher-forced from the final state against the oracle
+    import oracle
+    so = oracle.System(mesh, bc=0)
+    fin = [sol.get_state_unknowns(f) for f in (_lib.FEM_FIELD_U, _lib.FEM_FIELD_V, _lib.FEM_FIELD_A)]
+    sol.advance(1)
+    _, u1, v1, a1 = so.step(oracle.NEWMARK, [2e-4, 1.0, 0.25, 0.5], fin[0].copy(), fin[1].copy(), fin[2].copy(),
+                            tol=1e-14)
+    one = max(float(np.linalg.norm(sol.get_state_unknowns(f) - r) / np.linalg.norm(r))
+              for f, r in ((_lib.FEM_FIELD_U, u1), (_lib.FEM_FIELD_V, v1), (_lib.FEM_FIELD_A, a1)))
     out = contract_line(METRIC, n * steps / wall, "DOF·steps/s", steps, args.warmup, 1e3 * wall / steps,
                         "f32+f64 (FP32 CG, FP64 refinement residual and dots)",
                         {"workload": f"Newmark wave, jittered 1024x1024 lattice (N={n}, nnz={nnz}), c=1, "
@@ -360,10 +378,16 @@ def run_wave_c3(args):
                         solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
                                 "kernel_ms": st.ms, "iters_per_step": st.iterations / steps},
                         roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_step<1,float>",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                                  "kernel": "k_sphase<1,float,*> (streaming engine phases)"
+                                  if sol.options.engine == _lib.FEM_ENGINE_STREAM else "k_step<1,float>",
                                   "note": "~200 MB working set: mostly Infinity-Cache resident"},
-                        accuracy={"trajectory_rel_l2_final": float(np.linalg.norm(uu - ur) / np.linalg.norm(ur)),
-                                  "reference": "FP64 GPU CG, tol 1e-13"})
+                        accuracy={"one_step_rel_l2": one,
+                                  "one_step_reference": "oracle (C restatement, FP64 CG to 1e-14), one step from "
+                                                        "the final state (u, v, a)",
+                                  "trajectory_rel_l2_final": float(np.linalg.norm(uu - ur) / np.linalg.norm(ur)),
+                                  "trajectory_reference": "FP64 GPU CG, tol 1e-13 (the oracle takes ~0.7 s per "
+                                                          "step at this size)"})
     # the reference's own wave scheme (CPUSolver.cpp:108-121) on the same mesh
     ctx.equation = Equation.Wave
     ctx.parameters[Equation.Wave] = WaveParameters(2e-4, 1.0)
@@ -388,9 +412,10 @@ def run_wave_c3(args):
         dt_ = time.perf_counter() - t
         model, ncpu = cpu_info()
         out["cpu_baseline"] = {"value": n * ns / dt_, "unit": "DOF·steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{ns} steps of the reference's wave scheme (CPUSolver.cpp:108-121; "
-                                         f"Newmark has no reference counterpart) on the same mesh, FP32 "
-                                         f"Eigen-default CG port ({its} iterations); host: {model} ({ncpu} CPUs)"}
+                               "sample": f"{ns} steps of the reference's wave scheme (CPUSolver.cpp:108-121) on the "
+                                         f"same mesh, FP32 Eigen-default CG port ({its} iterations); a different "
+                                         f"scheme from the GPU line's Newmark, which has no reference "
+                                         f"counterpart; host: {model} ({ncpu} CPUs)"}
     return out
 
 
@@ -422,15 +447,21 @@ def c4_instances(ninst=256, which=None):
     return [(meshes.jittered_square(specs[i][0], seed=1000 + i),) + specs[i][1:] for i in idx]
 
 
-def run_batched_c4(args):
+def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0):
     """256 advection-diffusion instances of uneven size (about 1.3e7
     unknowns in all), 16 right-hand sides each, backward Euler dt = 1e-3,
-    unnormalised constant velocity per instance; one persistent launch for
-    all instances and steps, FP32 BiCGSTAB + FP64 refinement (tol 1e-10)."""
+    unnormalised constant velocity per instance, FP32 BiCGSTAB + FP64
+    refinement (tol 1e-10), autotuned between the persistent team plans and
+    the streaming engine.  `warmup` untimed steps, then `steps` timed steps
+    continuing the same trajectories.  Outside the timed region: one step of
+    a sample of instances and columns against the oracle (teacher-forced from
+    the final state), the same steps on the same engine in FP64, a sample of
+    the plain FP64 engine and of the FP32 CPU port."""
     from femhip import AdvectionDiffusionParameters, Batch, FEMContext, SolverOptions, _lib
-    steps = args.steps or 200
-    inst = c4_instances(args.instances)
-    log("batched: meshes generated")
+    ninst = args.instances
+    t_gen = time.perf_counter()
+    inst = c4_instances(ninst)
+    log("batched: meshes generated", round(time.perf_counter() - t_gen, 1), "s")
     ctxs, fields, prms = [], [], []
     tot_n = tot_nnz = 0
     for m, kap, vel, cen, sig in inst:
@@ -441,6 +472,7 @@ def run_batched_c4(args):
         ctxs.append(c)
         fields.append(np.ascontiguousarray(gauss_sum(X, cen, sig)))
         prms.append(AdvectionDiffusionParameters(1e-3, kap))
+    log("batched: contexts assembled, N =", tot_n)
 
     def make(**kw):
         kw.setdefault("predictor", PREDICTORS[args.predictor])
@@ -450,24 +482,29 @@ def run_batched_c4(args):
             b.set_equation(i, prms[i])
             b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
         return b
-    log("batched: contexts assembled, N =", tot_n)
+
+    def timed(b):
+        if warmup > 0:
+            b.advance(warmup)
+        b.sync()
+        t = time.perf_counter()
+        status = b.advance(steps)
+        b.sync()
+        return status, time.perf_counter() - t
+
     if args.engine == "stream":
-        b = make(tol=1e-10, engine=_lib.FEM_ENGINE_STREAM)
+        b = make(tol=args.tol, engine=_lib.FEM_ENGINE_STREAM)
         tuned = None
     else:
-        b = make(tol=1e-10)
+        b = make(tol=args.tol)
         tuned = b.autotune(2) if not args.no_autotune else None
+        for i in range(len(ctxs)):  # the autotuner restores the state; start from the initial fields
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
     log("batched: autotuned", tuned)
     engine = "stream" if b.options.engine == _lib.FEM_ENGINE_STREAM else "persistent"
-    for i in range(len(ctxs)):
-        b.set_state_unknowns(i, _lib.FEM_FIELD_U, fields[i])
-    b.sync()
-    t = time.perf_counter()
-    status = b.advance(steps)
-    b.sync()
-    wall = time.perf_counter() - t
+    status, wall = timed(b)
     st = b.last_stats
-    log("batched: timed run done", wall)
+    log("batched: timed run", round(wall, 3), "s")
     dof = 16 * tot_n
     # algorithmic bytes instance by instance: larger meshes iterate more
     alg = 0.0
@@ -475,50 +512,57 @@ def run_batched_c4(args):
         si = b.instance_stats(i)
         alg += step_bytes(c.num_unknowns(), c.info().nnz, 16, 1, si.iterations, si.sweeps, steps, bicgstab=True)
     ach = alg / (st.ms * 1e-3) / 1e9
-    finite = all(np.all(np.isfinite(b.get_state_unknowns(i))) for i in range(len(ctxs)))
-    out = contract_line(METRIC, dof * steps / wall, "DOF·steps/s", steps, 0, 1e3 * wall / steps,
-                        "f32+f64 (FP32 BiCGSTAB, FP64 refinement residual and dots)",
-                        {"workload": f"batched advection-diffusion, {len(ctxs)} jittered meshes of side "
-                                     f"log-U[32,512] (sum N={tot_n}, sum nnz={tot_nnz}), 16 RHS each, dt=1e-3, "
-                                     f"tol 1e-10", "global_batch": len(ctxs) * 16,
-                         "parallelism": f"single GPU, all instances per launch ({engine} engine"
-                                         f"{', autotuned' if tuned is not None else ''})"},
-                        status=int(status), all_finite=bool(finite),
-                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
-                                "kernel_ms": st.ms},
-                        roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                                  "kernel": "k_sphase<16,float,*> (stream engine phases)" if engine == "stream"
-                                  else "k_step<16,float,BiCGSTAB>"})
+    out = {"value": dof * steps / wall, "unit": "DOF·steps/s", "steps": steps, "warmup": warmup,
+           "ms_per_step": 1e3 * wall / steps, "status": int(status),
+           "workload": f"batched advection-diffusion, {len(ctxs)} jittered meshes of side log-U[32,512] "
+                       f"(sum N={tot_n}, sum nnz={tot_nnz}), 16 RHS each, dt=1e-3, tol {args.tol:g}; timed steps "
+                       f"{warmup + 1}..{warmup + steps}",
+           "engine": engine + (" (autotuned)" if tuned is not None else ""),
+           "solver": {"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
+                      "kernel_ms": st.ms},
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "k_sphase<16,float,*> (streaming engine phases)" if engine == "stream"
+                        else "k_step<16,float,BiCGSTAB>",
+                        "bytes_per_launch": alg,
+                        "note": "algorithmic bytes summed instance by instance (iterations and sweeps of each) "
+                                "over the device time of the advance"}}
     if not args.no_accuracy:
-        ref = make(precision=_lib.FEM_PREC_FP64, tol=1e-13, engine=_lib.FEM_ENGINE_STREAM)
-        for _ in range(steps // 10):
-            ref.advance(10)
-            log("batched: FP64 reference", ref.last_stats.steps)
-        if steps % 10:
-            ref.advance(steps % 10)
-        errs = []
-        for i in range(len(ctxs)):
-            a, r = b.get_state_unknowns(i), ref.get_state_unknowns(i)
-            errs.append(float(np.linalg.norm(a - r) / np.linalg.norm(r)))
-        # one step teacher-forced from the FP64 state: both solvers start from it
-        one = []
-        for i in range(len(ctxs)):
-            b.set_state_unknowns(i, _lib.FEM_FIELD_U, ref.get_state_unknowns(i))
+        # one step teacher-forced from the final state, against the oracle
+        # (FP64 BiCGSTAB to 1e-14) on the 2 smallest and the largest
+        # instances, 4 columns each
+        import oracle
+        order = np.argsort([c.num_unknowns() for c in ctxs], kind="stable")
+        sample = [int(order[0]), int(order[1]), int(order[-1])]
+        starts = {i: b.get_state_unknowns(i).reshape(-1, 16) for i in sample}
         b.advance(1)
-        ref.advance(1)
-        for i in range(len(ctxs)):
-            a, r = b.get_state_unknowns(i), ref.get_state_unknowns(i)
-            one.append(float(np.linalg.norm(a - r) / np.linalg.norm(r)))
-        out["accuracy"] = {"one_step_rel_l2_max_over_instances": float(np.max(one)),
-                           "trajectory_rel_l2_final_max_over_instances": float(np.max(errs)),
-                           "reference": "FP64 GPU (stream engine), tol 1e-13",
-                           "gate_1e-8_one_step": bool(np.max(one) <= 1e-8)}
-        ref.close()
+        errs = []
+        for i in sample:
+            m, kap, vel, cen, sig = inst[i]
+            s = oracle.System(m, bc=0, velocity=vel, normalize=False)
+            got = b.get_state_unknowns(i).reshape(-1, 16)
+            for j in (0, 5, 10, 15):
+                _, ref, _, _ = s.step(oracle.ADVDIFF, [1e-3, kap], starts[i][:, j].copy(), krylov=oracle.BICGSTAB,
+                                      tol=1e-14)
+                errs.append(float(np.linalg.norm(got[:, j] - ref) / np.linalg.norm(ref)))
+        out["accuracy"] = {"one_step_rel_l2_max": max(errs), "gate_1e-8": bool(max(errs) <= 1e-8),
+                           "sample": f"instances {sample} (2 smallest, largest), columns 0,5,10,15, one step from "
+                                     f"the final state",
+                           "reference": "oracle (C restatement, FP64 BiCGSTAB, tol 1e-14)"}
     b.close()
-    if not args.no_plain:
-        # one step of each instance in turn on the plain FP64 engine, for up to
-        # ~30 s of solver time (a sample when the batch does not fit)
+    if not args.no_fp64:
+        ref = make(tol=args.tol, precision=_lib.FEM_PREC_FP64,
+                   engine=b.options.engine if engine == "stream" else _lib.FEM_ENGINE_PERSISTENT)
+        _, w64 = timed(ref)
+        v64 = dof * steps / w64
+        out["fp64_same_engine"] = {"value": v64, "unit": "DOF·steps/s", "ms_per_step": 1e3 * w64 / steps,
+                                   "speedup_of_value": out["value"] / v64,
+                                   "note": f"the same {engine} engine and steps with FP64 Krylov vectors and matrix"}
+        ref.close()
+    if not args.no_plain and plain_seconds > 0:
+        # one step of each instance in turn on the plain FP64 engine (one
+        # right-hand side at a time, host-read reduction scalars), for up to
+        # plain_seconds of solver time
         done_dof, spent, count = 0, 0.0, 0
         for i in range(len(ctxs)):
             pl = Batch([ctxs[i]], SolverOptions(nrhs=16, engine=_lib.FEM_ENGINE_PLAIN,
@@ -533,18 +577,17 @@ def run_batched_c4(args):
             done_dof += 16 * ctxs[i].num_unknowns()
             count += 1
             pl.close()
-            log("batched: plain FP64 engine, instance", i, "cumulative s", round(spent, 2))
-            if spent > 30.0:
+            if spent > plain_seconds:
                 break
         pv = done_dof / spent
         out["plain_fp64_gpu"] = {"value": pv, "unit": "DOF·steps/s", "steps": 1,
                                  "sample": f"1 step of instances 0..{count - 1} ({done_dof} DOF), one right-hand "
                                            f"side at a time with host-read reduction scalars",
                                  "speedup_of_value": out["value"] / pv}
-    if not args.no_cpu:
+    if not args.no_cpu and cpu_seconds > 0:
         # the reference solves advection-diffusion with Eigen's CG, which can
         # stagnate up to its 2N iteration cap on these nonsymmetric systems:
-        # sample the smallest instance, one column at a time, for up to ~20 s
+        # sample the smallest instances, one column at a time
         import oracle
         order = np.argsort([c.num_unknowns() for c in ctxs])[:8]
         spent, dofs, its, used = 0.0, 0, 0, []
@@ -559,8 +602,7 @@ def run_batched_c4(args):
                 its += int(k_)
                 dofs += s.n
             used.append(int(i0))
-            log("batched: CPU port instance", int(i0), "N", s.n, "s", round(spent, 2))
-            if spent > 20.0:
+            if spent > cpu_seconds:
                 break
         model, ncpu = cpu_info()
         out["cpu_baseline"] = {"value": dofs / spent, "unit": "DOF·steps/s", "cores": 1, "kind": "port",
@@ -570,6 +612,18 @@ def run_batched_c4(args):
     for c in ctxs:
         c.close()
     return out
+
+
+def run_batched_c4(args):
+    """The batched line on its own (C4: 200 steps by default)."""
+    steps = args.steps or 200
+    r = batched_measure(args, steps, args.warmup, plain_seconds=30.0, cpu_seconds=20.0)
+    extra = {k: v for k, v in r.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step",
+                                                      "workload", "engine")}
+    return contract_line(METRIC, r["value"], r["unit"], steps, args.warmup, r["ms_per_step"],
+                         "f32+f64 (FP32 BiCGSTAB, FP64 refinement residual and dots)",
+                         {"workload": r["workload"], "global_batch": args.instances * 16,
+                          "parallelism": f"single GPU, all instances per launch ({r['engine']})"}, **extra)
 
 
 def run_spmv(args):
@@ -620,10 +674,14 @@ def main():
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")
     ap.add_argument("--inner-tol", type=float, default=None,
-                    help="FP32 sweep reduction (grayscott: 1e-5, batched: the library's 1e-4)")
+                    help="FP32 sweep reduction (grayscott: 1e-6, batched: the library's 1e-4)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--frames", type=int, default=200,
+                    help="grayscott: frames timed through the surface-space Solver path")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (counter calibration)")
     args = ap.parse_args()
@@ -649,11 +707,38 @@ def main():
         return
     if args.steps is None:
         args.steps = 2000
-
     if not args.no_spmv:
         # torch (device buffers of the SpMV leg) must load its HIP runtime
         # before the library does: a process holds one runtime
         import torch  # noqa: F401
+    out = grayscott_line(args)
+    if not args.no_spmv:
+        log("spmv")
+        out["spmv"] = spmv_measure()
+    if not args.no_adjoint:
+        log("adjoint")
+        out["adjoint"] = adjoint_measure()
+    if not args.no_batched:
+        log("batched")
+        out["batched"] = batched_measure(args, args.steps if args.steps <= 200 else 200, args.warmup)
+    if not args.no_cpu and args.cpu_steps > 0:
+        log("cpu baseline")
+        out["cpu_baseline"] = run_cpu_baseline(out.pop("_mesh"), out.pop("_u0"), out.pop("_v0"),
+                                               [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], args.cpu_steps)
+    for k in ("_mesh", "_u0", "_v0"):
+        out.pop(k, None)
+    print(json.dumps(out))
+
+
+def grayscott_line(args):
+    """The headline line: Gray-Scott on the icosphere refined args.level
+    times.  W = args.warmup untimed steps from the initial state, then
+    K = args.steps timed steps continuing the same trajectory in one launch.
+    Outside the timed region: the trajectory after W + K steps and one step
+    teacher-forced from the oracle's states at three points against the
+    oracle; the same steps on the same engine in FP64; the plain FP64 engine;
+    and frames through the surface-space Solver boundary."""
+    import oracle
     from femhip import Equation, FEMContext, HIPSolver, ReactionDiffusionParameters, SolverOptions, _lib, meshes
 
     mesh = meshes.icosphere(args.level)
@@ -662,11 +747,13 @@ def main():
     nnz = ctx.info().nnz
     u0, v0 = gs_initial(mesh, n)
     prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    oprm = [0.5, 2e-4, 1e-4, 0.037, 0.06, 1]
     ctx.equation = Equation.Reaction_Diffusion
     ctx.parameters[Equation.Reaction_Diffusion] = prm
+    W, K = args.warmup, args.steps
 
     def solver(**kw):
-        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol or 1e-5, team_size=args.team_size,
+        o = SolverOptions(tol=args.tol, inner_tol=args.inner_tol or 1e-6, team_size=args.team_size,
                           predictor=PREDICTORS[args.predictor])
         for k_, v_ in kw.items():
             setattr(o, k_, v_)
@@ -675,91 +762,90 @@ def main():
         s.set_state_unknowns(_lib.FEM_FIELD_V, v0)
         return s
 
+    def timed(s):
+        if W > 0:
+            s.advance(W)
+        s.sync()
+        t = time.perf_counter()
+        status = s.advance(K)
+        s.sync()
+        return status, time.perf_counter() - t
+
     log("grayscott: N =", n)
     sol = solver()
     tuned = None
     if args.team_size == 0 and not args.no_autotune:
-        tuned = sol.autotune(20)  # untimed; state restored
-    if args.warmup > 0:
-        sol.advance(args.warmup)
-    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
-    sol.set_state_unknowns(_lib.FEM_FIELD_V, v0)
-    sol.sync()
-    t0 = time.perf_counter()
-    status = sol.advance(args.steps)
-    sol.sync()
-    wall = time.perf_counter() - t0
+        tuned = sol.autotune(20)  # untimed; the state is restored
+    status, wall = timed(sol)
     log("grayscott: timed run", wall)
     st = sol.last_stats
     kernel_ms = st.ms
-    dof_steps = 2 * n * args.steps
-    value = dof_steps / wall
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "DOF·steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64 (FP32 CG, FP64 refinement residual and dots)",
-        "data": "synthetic",
-        "config": {
-            "workload": f"gray-scott icosphere L{args.level} (N={n}, nnz={nnz}, 2 species), dt=0.5, "
-                        f"Du=2e-4 Dv=1e-4 F=0.037 k=0.06, 50 patches, lumped reaction weighting, "
-                        f"tol {args.tol:g}",
-            "global_batch": 1,
-            "parallelism": "single GPU, persistent cooperative kernel",
-        },
-        "status": int(status),
-        "solver": {"iterations": int(st.iterations), "sweeps": int(st.sweeps),
-                   "team_size": sol.options.team_size, "autotune": tuned,
-                   "iters_per_solve": st.iterations / max(1, 2 * args.steps),
-                   "final_residual": st.residual, "kernel_ms": kernel_ms},
-    }
-    # roofline of the persistent stepping kernel (one launch = the timed region)
-    bytes_alg = step_bytes(n, nnz, 2, 2, st.iterations, st.sweeps, args.steps, fused=True)
+    value = 2 * n * K / wall
+    out = contract_line(
+        METRIC, value, "DOF·steps/s", K, W, 1e3 * wall / K, "f32+f64 (FP32 CG, FP64 refinement residual and dots)",
+        {"workload": f"gray-scott icosphere L{args.level} (N={n}, nnz={nnz}, 2 species), dt=0.5, Du=2e-4 Dv=1e-4 "
+                     f"F=0.037 k=0.06, 50 patches, lumped reaction weighting, tol {args.tol:g}; timed steps "
+                     f"{W + 1}..{W + K} of the trajectory in one launch",
+         "global_batch": 1, "parallelism": "single GPU, persistent cooperative kernel"},
+        status=int(status),
+        solver={"krylov_iterations": int(st.iterations), "sweeps": int(st.sweeps),
+                "krylov_iterations_per_step": st.iterations / K, "sweeps_per_step": st.sweeps / K,
+                "note": "one Krylov iteration advances the u and v solves together",
+                "team_size": sol.options.team_size, "autotune": tuned, "final_residual": st.residual,
+                "kernel_ms": kernel_ms})
+    # roofline of the persistent stepping kernel: the timed region is one launch
+    bytes_alg = step_bytes(n, nnz, 2, 2, st.iterations, st.sweeps, K, fused=True)
     ach = bytes_alg / (kernel_ms * 1e-3) / 1e9
-    tr = PROFILED_TRAFFIC["grayscott"]
+    prof = PROFILED_TRAFFIC["grayscott"]
+    same_cmd = (args.steps, args.warmup, args.level, args.tol) == (prof["steps"], prof["warmup"], 6, 1e-10)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": tr["bytes_per_step"] * args.steps,
-                       "traffic_source": tr["source"] + " (per-step bytes x steps)",
-                       "kernel": "k_step<2,float>", "bytes_per_launch": bytes_alg,
-                       "kernel_ms": kernel_ms}
+                       "frac": ach / HBM_PEAK_GBS,
+                       "traffic": prof["bytes_per_launch"] if same_cmd else None,
+                       "traffic_source": prof["source"] if same_cmd else "not profiled for these arguments",
+                       "traffic_note": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the timed dispatch: bytes between L2 "
+                                       "and the fabric (Infinity Cache hits included), scaled by a copy-kernel "
+                                       "calibration; the 8 MB working set stays cache-resident",
+                       "kernel": "k_step<2,float>", "bytes_per_launch": bytes_alg, "kernel_ms": kernel_ms}
+    uT = sol.get_state_unknowns(_lib.FEM_FIELD_U)
+    vT = sol.get_state_unknowns(_lib.FEM_FIELD_V)
 
     if not args.no_accuracy:
-        # FP64 reference trajectory (persistent FP64 CG, tol 1e-13) and
-        # teacher-forced one-step errors at 5 checkpoints
-        ref = solver(precision=_lib.FEM_PREC_FP64, tol=1e-13)
-        one = solver()
-        chk = max(1, args.steps // 5)
+        # the oracle trajectory (C restatement, FP64 CG to 1e-14) and one step
+        # teacher-forced from its states after W, W + K/2 and W + K steps
+        s = oracle.System(mesh, bc=1)
+        marks = sorted({W, W + K // 2, W + K})
+        u, v = u0.copy(), v0.copy()
+        one = []
         done = 0
-        one_step = []
-        while done < args.steps:
-            m = min(chk, args.steps - done)
-            ref.advance(m)
-            done += m
-            uu = ref.get_state_unknowns(_lib.FEM_FIELD_U)
-            vv = ref.get_state_unknowns(_lib.FEM_FIELD_V)
-            r1 = solver(precision=_lib.FEM_PREC_FP64, tol=1e-13)
-            for s_ in (one, r1):
-                s_.set_state_unknowns(_lib.FEM_FIELD_U, uu)
-                s_.set_state_unknowns(_lib.FEM_FIELD_V, vv)
-                s_.advance(1)
-            e = max(np.linalg.norm(one.get_state_unknowns(f) - r1.get_state_unknowns(f)) /
-                    np.linalg.norm(r1.get_state_unknowns(f)) for f in (_lib.FEM_FIELD_U, _lib.FEM_FIELD_V))
-            one_step.append(e)
-            r1.close()
-        traj = max(np.linalg.norm(sol.get_state_unknowns(f) - ref.get_state_unknowns(f)) /
-                   np.linalg.norm(ref.get_state_unknowns(f)) for f in (_lib.FEM_FIELD_U, _lib.FEM_FIELD_V))
-        out["accuracy"] = {"one_step_rel_l2_max": max(one_step), "one_step_rel_l2": one_step,
-                           "trajectory_rel_l2_final": traj, "reference": "FP64 GPU CG, tol 1e-13",
-                           "gate_1e-8_one_step": bool(max(one_step) <= 1e-8)}
-        ref.close()
-        one.close()
+        probe = solver()
+        for mk in marks:
+            if mk > done:
+                _, u, v, _ = s.run(oracle.GRAYSCOTT, oprm, mk - done, u, v, tol=1e-14)
+                done = mk
+            if mk == W + K:
+                traj = max(float(np.linalg.norm(uT - u) / np.linalg.norm(u)),
+                           float(np.linalg.norm(vT - v) / np.linalg.norm(v)))
+            probe.set_state_unknowns(_lib.FEM_FIELD_U, u)
+            probe.set_state_unknowns(_lib.FEM_FIELD_V, v)
+            probe.advance(1)
+            _, u1, v1, _ = s.step(oracle.GRAYSCOTT, oprm, u.copy(), v.copy(), tol=1e-14)
+            one.append(max(float(np.linalg.norm(probe.get_state_unknowns(_lib.FEM_FIELD_U) - u1) / np.linalg.norm(u1)),
+                           float(np.linalg.norm(probe.get_state_unknowns(_lib.FEM_FIELD_V) - v1) / np.linalg.norm(v1))))
+        probe.close()
+        out["accuracy"] = {"one_step_rel_l2_max": max(one), "one_step_rel_l2": one, "one_step_after_steps": marks,
+                           "trajectory_rel_l2_final": traj, "trajectory_steps": W + K,
+                           "reference": "oracle (C restatement of CPUSolver, FP64 CG to 1e-14)",
+                           "gate_1e-8_one_step": bool(max(one) <= 1e-8),
+                           "gate_1e-8_trajectory": bool(traj <= 1e-8)}
+
+    if not args.no_fp64:
+        f64 = solver(precision=_lib.FEM_PREC_FP64)
+        _, w64 = timed(f64)
+        v64 = 2 * n * K / w64
+        out["fp64_same_engine"] = {"value": v64, "unit": "DOF·steps/s", "ms_per_step": 1e3 * w64 / K,
+                                   "kernel": "k_step<2,double>", "speedup_of_value": value / v64,
+                                   "note": "the same persistent kernel and steps with FP64 Krylov vectors and matrix"}
+        f64.close()
 
     if not args.no_plain and args.plain_steps > 0:
         pl = solver(engine=_lib.FEM_ENGINE_PLAIN, precision=_lib.FEM_PREC_FP64)
@@ -770,21 +856,36 @@ def main():
         tw = time.perf_counter() - t
         pv = 2 * n * args.plain_steps / tw
         out["plain_fp64_gpu"] = {"value": pv, "unit": "DOF·steps/s", "steps": args.plain_steps,
-                                 "iterations": int(pl.last_stats.iterations),
-                                 "speedup_of_value": value / pv}
+                                 "iterations": int(pl.last_stats.iterations), "speedup_of_value": value / pv,
+                                 "note": "plain FP64 engine: one right-hand side at a time, separate kernels, "
+                                         "host-read reduction scalars"}
         pl.close()
 
-    if not args.no_spmv:
-        log("spmv")
-        out["spmv"] = spmv_measure()
-    if not args.no_adjoint:
-        log("adjoint")
-        out["adjoint"] = adjoint_measure()
-
-    if not args.no_cpu and args.cpu_steps > 0:
-        log("cpu baseline")
-        out["cpu_baseline"] = run_cpu_baseline(mesh, u0, v0, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], args.cpu_steps)
-    print(json.dumps(out))
+    if args.frames > 0:
+        # the reference's per-frame call pattern (Application.cpp:530-540):
+        # the surface values (v for Gray-Scott, float per vertex) go in, one
+        # step, the values come back; u stays on the device
+        fr = solver()
+        fr.advance(W)
+        surf = fr.get_state(_lib.FEM_FIELD_V)
+        fr.sync()
+        t = time.perf_counter()
+        its = 0
+        for _ in range(args.frames):
+            fr.set_state(_lib.FEM_FIELD_V, surf)
+            fr.advance(1)
+            its += fr.last_stats.iterations
+            surf = fr.get_state(_lib.FEM_FIELD_V)
+        tf = time.perf_counter() - t
+        out["frames"] = {"frames": args.frames, "ms_per_frame": 1e3 * tf / args.frames,
+                         "value": 2 * n * args.frames / tf, "unit": "DOF·steps/s",
+                         "krylov_iterations_per_frame": its / args.frames,
+                         "path": "HIPSolver.set_state(v) -> advance(1) -> get_state(v) per frame, through the C ABI"}
+        fr.close()
+    out["_mesh"], out["_u0"], out["_v0"] = mesh, u0, v0
+    sol.close()
+    ctx.close()
+    return out
 
 
 if __name__ == "__main__":

@@ -233,6 +233,7 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.max_sweeps = o.max_sweeps > 0 ? o.max_sweeps : 30;
   I.max_iters = o.max_iters > 0 ? o.max_iters : std::max(1, 2 * c->n);  // Eigen default 2N
   I.weighting = h.weighting;
+  I.max_row = c->max_row;
   const double tol = o.tol > 0 ? o.tol : 1e-10;
   const double itol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   I.tol2 = tol * tol;
@@ -1301,6 +1302,13 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
       const int g = std::max(1, std::min(maxb, (int)std::lround(g0 * f)));
       if (std::find(cands.begin(), cands.end(), g) == cands.end()) cands.push_back(g);
     }
+    // one and two workgroups per CU: no CU hosts one workgroup more than
+    // another, so no straggler holds up the barriers
+    int dev = 0, cus = 0;
+    FEM_HIP(hipGetDevice(&dev));
+    FEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int g : {cus / 2, cus, 2 * cus})
+      if (g >= 1 && g <= maxb && std::find(cands.begin(), cands.end(), g) == cands.end()) cands.push_back(g);
   } else {
     for (double f : {0.25, 0.5, 0.75, 1.0}) cands.push_back(std::max(1, (int)(maxb * f)));
   }

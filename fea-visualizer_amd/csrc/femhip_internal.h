@@ -56,7 +56,7 @@ struct Inst {
   int warm;                  // initial guess = previous primary variable
   int max_inner, max_sweeps, max_iters;
   int weighting;             // Gray-Scott reaction: 0 nodal (reference), 1 lumped mass
-  int pad_;
+  int max_row;               // longest row (nonzeros) of the pattern
   double tol2, inner_tol2;
   double dt, p1, p2, p3, p4; // equation parameters
   const int* rp;

@@ -182,6 +182,7 @@ int linear_solve_noalloc(fem_ctx* c, const double coef[3], const fem_solver_opts
   I.kv = 1;
   I.eq = EQ_SOLVE;
   I.krylov = kri;
+  I.max_row = c->max_row;
   I.warm = o.warm_start;
   I.max_inner = o.max_inner > 0 ? o.max_inner : 200;
   I.max_sweeps = o.max_sweeps > 0 ? o.max_sweeps : 30;

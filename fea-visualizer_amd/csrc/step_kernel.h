@@ -836,9 +836,13 @@ __device__ void phase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, const T
 // residual pass, which also takes its column indices).
 // ---------------------------------------------------------------------------
 
-// Fill the cache for this workgroup's rows; true (uniform) when every row fits.
+// Fill the cache for this workgroup's rows.  Whether the team caches is
+// decided from team-wide quantities -- the rows every workgroup is dealt and
+// the instance's longest row -- so all workgroups of a team agree: with the
+// cache the pipelined CG starts inside the update phase, one barrier fewer,
+// and a team whose workgroups disagreed would wait at different barriers.
 template <int K, typename TI>
-__device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, RowCache<TI>& RC) {
+__device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, int rows_per_wg, RowCache<TI>& RC) {
   constexpr int RPP = kBlock / K;
   const int jr = threadIdx.x % K;
   const int kv = I.kv, vj = (kv == 1) ? 0 : jr;
@@ -846,7 +850,7 @@ __device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, RowCache<TI>& 
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
-  int ok = (row_hi - row_lo + RPP - 1) / RPP <= kCacheP;
+  const bool ok = (rows_per_wg + RPP - 1) / RPP <= kCacheP && I.max_row <= kCacheE;
   if (ok) {
     for (int q = 0; q < kCacheP; q++) {
       const int row = row_lo + (int)threadIdx.x / K + q * RPP;
@@ -854,7 +858,6 @@ __device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, RowCache<TI>& 
       if (row < row_hi) {
         lo = rp[row];
         hi = rp[row + 1];
-        if (hi - lo > kCacheE) ok = 0;
       }
       for (int e = 0; e < kCacheE; e++) {
         const int s = lo + e;
@@ -865,7 +868,8 @@ __device__ bool fill_cache(const Inst& I, int row_lo, int row_hi, RowCache<TI>& 
       }
     }
   }
-  return __syncthreads_and(ok) != 0;
+  __syncthreads();
+  return ok;
 }
 
 // Pipelined CG start from the cache (same arithmetic as phase_pcg_init).
@@ -1460,7 +1464,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
   const int row_lo = min(n, rank * rows_per_wg);
   const int row_hi = min(n, row_lo + rows_per_wg);
   bool cached = false;
-  if constexpr (KR != 2) cached = fill_cache<K, TI>(I, row_lo, row_hi, RC);
+  if constexpr (KR != 2) cached = fill_cache<K, TI>(I, row_lo, row_hi, rows_per_wg, RC);
   long long its_total = 0, sweeps_total = 0;
   int status = 0;
   TI* r0 = (TI*)I.r;

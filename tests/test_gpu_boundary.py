@@ -156,3 +156,66 @@ def test_grayscott_surface_is_v(gpu):
     sol.set_state(_lib.FEM_FIELD_V, vs)
     assert np.array_equal(sol.get_state_unknowns(_lib.FEM_FIELD_U), u)
     assert np.array_equal(sol.get_state(_lib.FEM_FIELD_V), vs)
+
+
+def fan_disc(rings=40, per_ring=16):
+    """Disc of concentric rings of per_ring vertices around one centre vertex
+    whose row holds per_ring + 1 entries (longer than the LDS row cache's
+    10), every other row 7."""
+    xyz = [(0.0, 0.0, 0.0)]
+    for i in range(1, rings + 1):
+        for j in range(per_ring):
+            t = 2 * np.pi * (j + 0.5 * (i % 2)) / per_ring
+            xyz.append((i / rings * np.cos(t), i / rings * np.sin(t), 0.0))
+    tri = [(0, 1 + j, 1 + (j + 1) % per_ring) for j in range(per_ring)]
+    for i in range(1, rings):
+        a, b = 1 + (i - 1) * per_ring, 1 + i * per_ring
+        for j in range(per_ring):
+            j1 = (j + 1) % per_ring
+            tri.append((a + j, b + j, b + j1) if i % 2 else (a + j, b + j, a + j1))
+            tri.append((a + j, b + j1, a + j1) if i % 2 else (a + j1, b + j, b + j1))
+    return meshes._pack(np.array(xyz), np.array(tri))
+
+
+@pytest.mark.parametrize("exchange", [_lib.FEM_EXCHANGE_COHERENT, _lib.FEM_EXCHANGE_FENCED])
+def test_team_where_only_last_workgroup_fits_row_cache(gpu, exchange):
+    """Gray-Scott on the icosphere refined 6 times with a team of 128
+    workgroups: every workgroup is dealt 321 rows (more than the LDS row
+    cache holds) but the last only 195.  The whole team must take the same
+    uncached path (the cached one starts the CG one barrier earlier); steps
+    inside one launch use the fused update, where the difference shows."""
+    import bench
+    m = meshes.icosphere(6)
+    s = oracle.System(m, bc=1)
+    u0, v0 = bench.gs_initial(m, s.n)
+    ctx = FEMContext(m, 1)
+    ctx.equation = Equation.Reaction_Diffusion
+    ctx.parameters[Equation.Reaction_Diffusion] = ReactionDiffusionParameters(
+        0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    sol = HIPSolver(ctx, SolverOptions(team_size=128, exchange=exchange, inner_tol=1e-6))
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, v0)
+    print("case team 128 on ico6, exchange", exchange, flush=True)
+    assert sol.advance(4) == _lib.FEM_OK
+    _, u, v, _ = s.run(oracle.GRAYSCOTT, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], 4, u0.copy(), v0.copy(), tol=1e-14)
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_U), u) <= 1e-8
+    assert rel(sol.get_state_unknowns(_lib.FEM_FIELD_V), v) <= 1e-8
+
+
+def test_long_row_in_one_workgroup(gpu):
+    """A fan mesh whose centre row (17 entries) exceeds the row cache while
+    every other row fits: the team of several workgroups runs uncached as a
+    whole.  Heat (pipelined CG), several steps per launch, against the
+    oracle."""
+    m = fan_disc()
+    s = oracle.System(m, bc=0)
+    assert int(np.diff(s.row_ptr).max()) == 17
+    x = m["xyz"].astype(np.float64)
+    u0 = s.gather(np.exp(-((x[:, 0] - 0.1) ** 2 + x[:, 1] ** 2) / 0.05))
+    ctx, sol = heat_solver(m)
+    assert ctx.num_max_nonzeros_per_row() == 17
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
+    print("case fan mesh, n", s.n, flush=True)
+    assert sol.advance(5) == _lib.FEM_OK
+    _, u, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], 5, u0.copy(), tol=1e-14)
+    assert rel(sol.get_state_unknowns(), u) <= 1e-8
