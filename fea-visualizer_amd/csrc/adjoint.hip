@@ -139,7 +139,9 @@ int adjoint_set_equation(fem_adjoint* a, int eq, const void* params, size_t size
   return 0;
 }
 
-int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, double* uT, fem_step_stats* stats) {
+// dev: u0 and uT are device pointers (copied device to device), else host.
+int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, double* uT, fem_step_stats* stats,
+                    bool dev) {
   if (a->eq < 0) {
     set_error("adjoint: set_equation first");
     return FEM_ERR_STATE;
@@ -163,7 +165,8 @@ int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, dou
   }
   a->have_forward = false;
   if (n > 0) {
-    FEM_HIP(hipMemcpyAsync(a->d_ckpt, u0, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    FEM_HIP(hipMemcpyAsync(a->d_ckpt, u0, sizeof(double) * n, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           st));
     FEM_HIP(hipMemcpyAsync(state_u(a->fwd), a->d_ckpt, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     FEM_HIP(hipStreamSynchronize(st));
   }
@@ -175,7 +178,8 @@ int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, dou
   h.traj = nullptr;
   if (r < 0) return r;
   if (uT && n > 0) {
-    FEM_HIP(hipMemcpyAsync(uT, state_u(a->fwd), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    FEM_HIP(hipMemcpyAsync(uT, state_u(a->fwd), sizeof(double) * n,
+                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     FEM_HIP(hipStreamSynchronize(st));
   }
   a->nsteps = nsteps;
@@ -185,7 +189,8 @@ int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, dou
   return r;
 }
 
-int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, fem_step_stats* stats) {
+// dev: gT, gu0 and gk are device pointers, else host.
+int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, fem_step_stats* stats, bool dev) {
   if (!a->have_forward || a->fwd_version != a->ctx->version) {
     set_error("adjoint_backward: run adjoint_forward (after the last assembly) first");
     return FEM_ERR_STATE;
@@ -235,7 +240,8 @@ int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, 
   InstHost& hf = a->fwd->insts[0];
   if (n > 0) {
     // mu_T = S^-T g: one plain solve on the transposed instance
-    FEM_HIP(hipMemcpyAsync(ha.d_b, gT, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    FEM_HIP(hipMemcpyAsync(ha.d_b, gT, sizeof(double) * n, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           st));
     FEM_HIP(hipMemsetAsync(ha.d_x, 0, sizeof(double) * n, st));
     FEM_HIP(hipStreamSynchronize(st));
     ha.eq_override = EQ_SOLVE;
@@ -282,9 +288,13 @@ int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, 
     hipLaunchKernelGGL(k_mass_mv, dim3(div_up(n, kBlock)), dim3(kBlock), 0, st, (int)n, c->d_rp, c->d_ci, c->d_M,
                        a->dt, ha.d_u, a->d_tmp);
     FEM_HIP(hipGetLastError());
-    if (gu0) FEM_HIP(hipMemcpyAsync(gu0, a->d_tmp, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (gu0)
+      FEM_HIP(hipMemcpyAsync(gu0, a->d_tmp, sizeof(double) * n, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             st));
   }
-  if (gk && c->ntri > 0) FEM_HIP(hipMemcpyAsync(gk, a->d_gk, sizeof(double) * c->ntri, hipMemcpyDeviceToHost, st));
+  if (gk && c->ntri > 0)
+    FEM_HIP(hipMemcpyAsync(gk, a->d_gk, sizeof(double) * c->ntri,
+                           dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
   FEM_HIP(hipStreamSynchronize(st));
   const double ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

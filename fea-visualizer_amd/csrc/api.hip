@@ -481,6 +481,83 @@ int fem_solver_smooth_normals(fem_solver* s, const float* normals_nv3, float ext
   })
 }
 
+// ------------------------------------------------------------------ surface
+
+int fem_surface_from_obj(const char* path, fem_surface** out) {
+  FEM_GUARD({
+    if (!path || !out) {
+      set_error("fem_surface_from_obj: null argument");
+      return FEM_ERR_INVALID;
+    }
+    *out = nullptr;
+    return surface_from_obj(path, out);
+  })
+}
+
+int fem_surface_get(const fem_surface* s, fem_mesh* mesh, const float** normals_nv3, int32_t* num_boundary_points) {
+  FEM_GUARD({
+    if (!s) {
+      set_error("fem_surface_get: null surface");
+      return FEM_ERR_INVALID;
+    }
+    return surface_get(s, mesh, normals_nv3, num_boundary_points);
+  })
+}
+
+int fem_surface_destroy(fem_surface* s) {
+  FEM_GUARD({
+    surface_free(s);
+    return FEM_OK;
+  })
+}
+
+int fem_export_ply(const fem_mesh* mesh, const float* normals_nv3, const float* values_nv, float vertex_extrusion,
+                   float threshold, int32_t mesh_type, const char* path) {
+  FEM_GUARD({
+    if (!mesh || !path || mesh->num_vertices < 0 || mesh->num_triangles < 0 ||
+        (mesh->num_vertices > 0 && (!mesh->xyz || !normals_nv3 || !values_nv)) ||
+        (mesh->num_triangles > 0 && !mesh->triangles)) {
+      set_error("fem_export_ply: invalid argument");
+      return FEM_ERR_INVALID;
+    }
+    return ply_export(mesh->xyz, mesh->num_vertices, mesh->triangles, mesh->num_triangles, normals_nv3, values_nv,
+                      vertex_extrusion, threshold, mesh_type, path);
+  })
+}
+
+int fem_solver_export_ply(fem_solver* s, const float* normals_nv3, float vertex_extrusion, float threshold,
+                          int32_t mesh_type, const char* path) {
+  FEM_GUARD({
+    if (!s || !path || !normals_nv3) {
+      set_error("fem_solver_export_ply: null argument");
+      return FEM_ERR_INVALID;
+    }
+    fem_batch* b = s->batch;
+    fem_ctx* c = b->insts[0].ctx;
+    const size_t nv = (size_t)c->nv, nt = (size_t)c->ntri;
+    std::vector<float> xyz(3 * nv), vals(nv);
+    std::vector<uint32_t> tri(3 * nt);
+    if (nv > 0) {
+      int rc = 0;
+      float* d = dalloc<float>(nv, &rc);
+      if (!d) return rc;
+      rc = batch_surface_values(b, 0, d);
+      hipError_t e = hipSuccess;
+      if (!rc) e = hipMemcpyAsync(vals.data(), d, sizeof(float) * nv, hipMemcpyDeviceToHost, b->stream);
+      if (!rc && e == hipSuccess)
+        e = hipMemcpyAsync(xyz.data(), c->d_xyz, sizeof(float) * 3 * nv, hipMemcpyDeviceToHost, b->stream);
+      if (!rc && e == hipSuccess && nt > 0)
+        e = hipMemcpyAsync(tri.data(), c->d_tri, sizeof(uint32_t) * 3 * nt, hipMemcpyDeviceToHost, b->stream);
+      if (!rc && e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      (void)hipFree(d);
+      if (rc) return rc;
+      if (e != hipSuccess) return hip_fail(e, "fem_solver_export_ply");
+    }
+    return ply_export(xyz.data(), (int32_t)nv, tri.data(), (int32_t)nt, normals_nv3, vals.data(), vertex_extrusion,
+                      threshold, mesh_type, path);
+  })
+}
+
 // ----------------------------------------------------------------- operator
 
 int fem_operator_create(fem_ctx* ctx, const double coef[3], int32_t precision, int32_t transpose,
@@ -602,7 +679,18 @@ int fem_adjoint_forward(fem_adjoint* a, const double* u0, int32_t nsteps, int32_
       set_error("null adjoint");
       return FEM_ERR_INVALID;
     }
-    return adjoint_forward(a, u0, nsteps, checkpoint_every, uT, stats);
+    return adjoint_forward(a, u0, nsteps, checkpoint_every, uT, stats, false);
+  })
+}
+
+int fem_adjoint_forward_device(fem_adjoint* a, const double* u0, int32_t nsteps, int32_t checkpoint_every,
+                               double* uT, fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!a) {
+      set_error("null adjoint");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_forward(a, u0, nsteps, checkpoint_every, uT, stats, true);
   })
 }
 
@@ -613,7 +701,18 @@ int fem_adjoint_backward(fem_adjoint* a, const double* grad_uT, double* grad_u0,
       set_error("null adjoint");
       return FEM_ERR_INVALID;
     }
-    return adjoint_backward(a, grad_uT, grad_u0, grad_kappa_e, stats);
+    return adjoint_backward(a, grad_uT, grad_u0, grad_kappa_e, stats, false);
+  })
+}
+
+int fem_adjoint_backward_device(fem_adjoint* a, const double* grad_uT, double* grad_u0, double* grad_kappa_e,
+                                fem_step_stats* stats) {
+  FEM_GUARD({
+    if (!a) {
+      set_error("null adjoint");
+      return FEM_ERR_INVALID;
+    }
+    return adjoint_backward(a, grad_uT, grad_u0, grad_kappa_e, stats, true);
   })
 }
 

@@ -26,6 +26,7 @@ namespace femhip {
 int adjoint_create(fem_ctx* ctx, const fem_solver_opts* opts, fem_adjoint** out);
 void adjoint_free(fem_adjoint* a);
 int adjoint_set_equation(fem_adjoint* a, int eq, const void* params, size_t size);
-int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, double* uT, fem_step_stats* stats);
-int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, fem_step_stats* stats);
+int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, double* uT, fem_step_stats* stats,
+                    bool dev);
+int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, fem_step_stats* stats, bool dev);
 }  // namespace femhip

@@ -167,6 +167,11 @@ int linear_solve(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, 
                  fem_step_stats* stats);
 int form_system(fem_ctx* c, const double coef[3], int fp32, double* S64, void* Si, double* dinv64, void* dinvi,
                 hipStream_t st);
+int ply_export(const float* xyz, int32_t nv, const uint32_t* tri, int32_t ntri, const float* normals,
+               const float* values, float vertex_extrusion, float threshold, int32_t mesh_type, const char* path);
+int surface_from_obj(const char* path, fem_surface** out);
+void surface_free(fem_surface* s);
+int surface_get(const fem_surface* s, fem_mesh* mesh, const float** normals, int32_t* num_boundary);
 int linear_solve_noalloc(fem_ctx* c, const double coef[3], const fem_solver_opts* opts, const double* b, double* x,
                          fem_step_stats* stats);
 }  // namespace femhip

@@ -59,6 +59,9 @@ FEM_PREDICTOR_PREVIOUS = 1
 FEM_PREDICTOR_QUADRATIC = 2
 FEM_PREDICTOR_CUBIC = 3
 FEM_PREDICTOR_QUARTIC = 4
+FEM_MESH_OPEN = 0
+FEM_MESH_CLOSED = 1
+FEM_MESH_MIRRORED = 2
 FEM_SPMV_STREAM = 32
 FEM_SPMV_STREAM_NT = 33
 SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)
@@ -180,6 +183,10 @@ SIGNATURES = [
     ("fem_ctx_solve", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
     ("fem_ctx_solve_noalloc", C.c_int, [P, PD, C.POINTER(fem_solver_opts), PD, PD, C.POINTER(fem_step_stats)]),
     ("fem_ctx_smooth_normals", C.c_int, [P, PF, PF, C.c_float, PF]),
+    ("fem_surface_from_obj", C.c_int, [C.c_char_p, C.POINTER(P)]),
+    ("fem_surface_get", C.c_int, [P, C.POINTER(fem_mesh), C.POINTER(PF), PI]),
+    ("fem_surface_destroy", C.c_int, [P]),
+    ("fem_export_ply", C.c_int, [C.POINTER(fem_mesh), PF, PF, C.c_float, C.c_float, I32, C.c_char_p]),
     ("fem_solver_create", C.c_int, [P, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_solver_destroy", C.c_int, [P]),
     ("fem_solver_set_equation", C.c_int, [P, I32, P, C.c_size_t]),
@@ -193,6 +200,7 @@ SIGNATURES = [
     ("fem_solver_state_device_ptr", C.c_int, [P, I32, C.POINTER(P)]),
     ("fem_solver_sync", C.c_int, [P]),
     ("fem_solver_smooth_normals", C.c_int, [P, PF, C.c_float, PF]),
+    ("fem_solver_export_ply", C.c_int, [P, PF, C.c_float, C.c_float, I32, C.c_char_p]),
     ("fem_solver_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_batch_create", C.c_int, [C.POINTER(P), I32, C.POINTER(fem_solver_opts), C.POINTER(P)]),
     ("fem_batch_destroy", C.c_int, [P]),
@@ -219,6 +227,8 @@ SIGNATURES = [
     ("fem_adjoint_set_equation", C.c_int, [P, I32, P, C.c_size_t]),
     ("fem_adjoint_forward", C.c_int, [P, PD, I32, I32, PD, C.POINTER(fem_step_stats)]),
     ("fem_adjoint_backward", C.c_int, [P, PD, PD, PD, C.POINTER(fem_step_stats)]),
+    ("fem_adjoint_forward_device", C.c_int, [P, P, I32, I32, P, C.POINTER(fem_step_stats)]),
+    ("fem_adjoint_backward_device", C.c_int, [P, P, P, P, C.POINTER(fem_step_stats)]),
 ]
 
 _lib = None
@@ -247,6 +257,18 @@ def load(path: str | None = None):
     if path is None:
         _lib = lib
     return lib
+
+
+def hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process.  A
+    library loaded before torch binds /opt/rocm's runtime and torch then maps
+    its own copy: two runtimes, whose device pointers the other does not
+    know.  Loaded after torch, the library binds torch's copy (same soname)."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({l.split()[-1] for l in f if "libamdhip64" in l})
+    except OSError:
+        return []
 
 
 def last_error() -> str:

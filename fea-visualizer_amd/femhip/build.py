@@ -21,7 +21,7 @@ INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(HERE, "libfemhip.so")
 
 SOURCES = ["api.hip", "scan.hip", "ctx.hip", "batch.hip", "step.hip", "step_plain.hip", "step_coherent.hip",
-           "step_fenced.hip", "stream.hip", "plain.hip", "adjoint.hip", "operator.hip", "gl_compat.hip", "noalloc.hip"]
+           "step_fenced.hip", "stream.hip", "plain.hip", "adjoint.hip", "operator.hip", "gl_compat.hip", "noalloc.hip", "surface.cpp"]
 
 # -packed-fp32-ops: keep v_pk_*_f32 out of every code object.
 HIPCC_FLAGS = [

@@ -1,4 +1,4 @@
-"""Surface meshes for the FEM backend: synthetic generators and an OBJ reader.
+"""Surface meshes for the FEM backend: synthetic generators and the OBJ reader.
 
 A mesh is a dict with ``xyz`` (float32 [nv, 3]), ``tri`` (uint32 [T, 3]) and
 ``on_boundary`` (uint8 [nv]) -- the fields the reference's FEMContext reads
@@ -140,66 +140,33 @@ def icosphere(level: int, radius: float = 1.0):
     return _pack(V * radius, F)
 
 
-def read_obj(path: str, quad_split: str = "shorter_diagonal"):
-    """OBJ reader following Surface::init_from_obj (src/Utils/Surface.cpp:57-133).
+def read_obj(path: str):
+    """Surface::init_from_obj (src/Utils/Surface.cpp:57-133) through the
+    library's reader (fem_surface_from_obj): the mesh dict plus the vertex
+    ``normals`` (float32 [nv, 3]).  Faces are triangulated (quads along the
+    shorter diagonal, larger polygons fanned), every object repeats the
+    vertex list as the reference does, boundary by the edge-count rule."""
+    import ctypes as C
 
-    * faces are triangulated (tinyobj ``triangulate = true``): triangles kept,
-      quads split by ``quad_split`` ("shorter_diagonal": the tinyobjloader rule
-      of recent releases, unpinned here because tinyobjloader is absent;
-      "fan": (0,1,2),(0,2,3)), larger polygons fanned;
-    * every shape re-appends the whole vertex list while its triangle indices
-      stay unshifted (Surface.cpp:77-80), so a file with S shapes yields
-      S copies of the vertices, the later ones isolated;
-    * the file must carry normals (Surface.cpp:71-72), as the reference demands;
-    * boundary by the edge-count rule (Surface.cpp:104-127).
-    """
-    verts = []
-    normals = 0
-    shapes = [[]]
-    with open(path, "r") as f:
-        for line in f:
-            if not line.strip() or line.startswith("#"):
-                continue
-            parts = line.split()
-            tag = parts[0]
-            if tag == "v":
-                verts.append([float(parts[1]), float(parts[2]), float(parts[3])])
-            elif tag == "vn":
-                normals += 1
-            elif tag in ("o", "g"):
-                if shapes[-1]:
-                    shapes.append([])
-            elif tag == "f":
-                idx = []
-                for p in parts[1:]:
-                    vi = int(p.split("/")[0])
-                    idx.append(vi - 1 if vi > 0 else len(verts) + vi)
-                shapes[-1].append(idx)
-    shapes = [s for s in shapes if s]
-    if not verts:
-        raise ValueError(f"File {path} does not contain a valid .obj mesh.")
-    if normals == 0:
-        raise ValueError(f"File {path} does not contain normals!")
-    V = np.asarray(verts, dtype=np.float32)
-    tris = []
-    for faces in shapes:
-        for idx in faces:
-            if len(idx) == 3:
-                tris.append(idx)
-            elif len(idx) == 4 and quad_split == "shorter_diagonal":
-                p = V[idx].astype(np.float64)
-                d02 = ((p[2] - p[0]) ** 2).sum()
-                d13 = ((p[3] - p[1]) ** 2).sum()
-                if d02 < d13:
-                    tris += [[idx[0], idx[1], idx[2]], [idx[0], idx[2], idx[3]]]
-                else:
-                    tris += [[idx[0], idx[1], idx[3]], [idx[1], idx[2], idx[3]]]
-            else:
-                for q in range(1, len(idx) - 1):
-                    tris.append([idx[0], idx[q], idx[q + 1]])
-    xyz = np.concatenate([V] * max(1, len(shapes)))
-    tri = np.asarray(tris, dtype=np.uint32).reshape(-1, 3)
-    return _pack(xyz, tri)
+    from . import _lib as L
+    lib = L.load()
+    h = C.c_void_p()
+    L.check(lib.fem_surface_from_obj(str(path).encode(), C.byref(h)))
+    try:
+        m = L.fem_mesh()
+        nrm = C.POINTER(C.c_float)()
+        nb = C.c_int32()
+        L.check(lib.fem_surface_get(h, C.byref(m), C.byref(nrm), C.byref(nb)))
+        nv, nt = m.num_vertices, m.num_triangles
+        xyz = np.ctypeslib.as_array(m.xyz, shape=(nv, 3)).copy() if nv else np.zeros((0, 3), np.float32)
+        tri = np.ctypeslib.as_array(m.triangles, shape=(nt, 3)).copy() if nt else np.zeros((0, 3), np.uint32)
+        onb = np.ctypeslib.as_array(m.on_boundary, shape=(nv,)).copy() if nv else np.zeros(0, np.uint8)
+        normals = np.ctypeslib.as_array(nrm, shape=(nv, 3)).copy() if nv else np.zeros((0, 3), np.float32)
+    finally:
+        lib.fem_surface_destroy(h)
+    out = _pack(xyz, tri, onb)
+    out["normals"] = normals
+    return out
 
 
 def mesh_stats(mesh, bc: int = 0):

@@ -354,6 +354,14 @@ class HIPSolver:
                                                     L.ptr(out, C.c_float)))
         return out
 
+    def export_ply(self, path: str, normals, vertex_extrusion: float = 0.25, threshold: float = 0.0,
+                   mesh_type: int = L.FEM_MESH_OPEN):
+        """Surface::export_to_ply of the current field, read from the device
+        (fem_solver_export_ply); normals: the mesh's vertex normals."""
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        L.check(self._lib.fem_solver_export_ply(self.handle, L.ptr(nrm, C.c_float), float(vertex_extrusion),
+                                                float(threshold), int(mesh_type), str(path).encode()))
+
     def init_newmark_acceleration(self):
         """Consistent initial acceleration of the Newmark wave from the current
         u: M a = -c^2 K u (both products and the solve on the GPU)."""
@@ -564,6 +572,17 @@ class Adjoint:
         L.check(self._lib.fem_adjoint_backward(self.handle, L.ptr(g, C.c_double), L.ptr(gu, C.c_double),
                                                L.ptr(gk, C.c_double), C.byref(self.backward_stats)))
         return gu, gk
+
+    def forward_device(self, u0_ptr: int, uT_ptr: int, nsteps: int, checkpoint_every: int = 10):
+        """forward on device pointers of FP64 [N] arrays (inputs complete)."""
+        L.check(self._lib.fem_adjoint_forward_device(self.handle, C.c_void_p(u0_ptr), int(nsteps),
+                                                     int(checkpoint_every), C.c_void_p(uT_ptr),
+                                                     C.byref(self.forward_stats)))
+
+    def backward_device(self, grad_uT_ptr: int, grad_u0_ptr: int, grad_kappa_ptr: int):
+        """backward on device pointers (FP64 [N], [N], [num_triangles])."""
+        L.check(self._lib.fem_adjoint_backward_device(self.handle, C.c_void_p(grad_uT_ptr), C.c_void_p(grad_u0_ptr),
+                                                      C.c_void_p(grad_kappa_ptr), C.byref(self.backward_stats)))
 
     def close(self):
         if getattr(self, "handle", None):

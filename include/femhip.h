@@ -224,6 +224,37 @@ int fem_ctx_solve_noalloc(fem_ctx* ctx, const double coef[3], const fem_solver_o
 int fem_ctx_smooth_normals(fem_ctx* ctx, const float* normals_nv3, const float* values_nv, float extrusion,
                            float* out_nv3);
 
+/* ---- surface input and output: host code around the GPU path ----
+ * fem_surface_from_obj <- Surface::init_from_obj (src/Utils/Surface.cpp:57-133):
+ *   faces triangulated (triangles kept; quads split along the shorter
+ *   diagonal, squared lengths in float; larger polygons fanned from their
+ *   first corner); every object or group appends the whole vertex list again
+ *   while its face indices keep pointing at the first copy (Surface.cpp:77-80);
+ *   vertex normals = glm::normalize of the summed corner normals
+ *   (Surface.cpp:89-102, NaN for a vertex no face uses); boundary = endpoints
+ *   of edges used by one triangle (Surface.cpp:104-127).  A missing file, a
+ *   file without vertices or without normals is refused with FEM_ERR_INVALID
+ *   and the reference's message.
+ * fem_export_ply <- Surface::export_to_ply (src/Utils/Surface.cpp:138-285):
+ *   the surface extruded along its normals by value * max(0, extrusion),
+ *   clipped where values fall below threshold + 1e-9, closed by a projected
+ *   copy for FEM_MESH_CLOSED / _MIRRORED, positions de-duplicated in first-use
+ *   order with glm::vec3 equality, coloured by the Viridis polynomial
+ *   (AssetStorage.cpp:141-152), written as ASCII PLY with the reference's
+ *   header and shortest round-trip numbers. */
+enum { FEM_MESH_OPEN = 0, FEM_MESH_CLOSED = 1, FEM_MESH_MIRRORED = 2 };  /* MeshType (Surface.hpp:25-29) */
+typedef struct fem_surface fem_surface;
+int fem_surface_from_obj(const char* path, fem_surface** out);
+/* Borrowed views of the surface's arrays, valid until fem_surface_destroy:
+ * the mesh to pass to fem_ctx_create, vertex normals [nv][3], boundary count
+ * (Surface::num_boundary_points).  Any output pointer may be NULL. */
+int fem_surface_get(const fem_surface* s, fem_mesh* mesh, const float** normals_nv3, int32_t* num_boundary_points);
+int fem_surface_destroy(fem_surface* s);
+/* mesh: positions and triangles (on_boundary unused); normals and values per
+ * vertex ([nv][3], [nv]). */
+int fem_export_ply(const fem_mesh* mesh, const float* normals_nv3, const float* values_nv, float vertex_extrusion,
+                   float threshold, int32_t mesh_type, const char* path);
+
 /* ---- solver: the Solver interface (Solver.hpp:8-15) ----
  * fem_solver_advance  <- Solver::advance_time        (CPUSolver.cpp:64-147)
  * fem_solver_clear    <- Solver::clear_values        (CPUSolver.cpp:54-59)
@@ -257,6 +288,12 @@ int fem_solver_sync(fem_solver* s);
  * boundary vertices 0.  normals_nv3: the mesh's vertex normals [nv][3];
  * out_nv3 receives the unnormalised sums [nv][3]. */
 int fem_solver_smooth_normals(fem_solver* s, const float* normals_nv3, float extrusion, float* out_nv3);
+/* PLY export of the solver's current surface field (u, or v for Gray-Scott,
+ * read from the device; boundary vertices 0 as after the frame's scatter,
+ * CPUSolver.cpp:28-37) on the context's mesh: fem_export_ply with the
+ * caller's vertex normals [nv][3]. */
+int fem_solver_export_ply(fem_solver* s, const float* normals_nv3, float vertex_extrusion, float threshold,
+                          int32_t mesh_type, const char* path);
 /* Autotuner (no reference counterpart): runs probe_steps under a few launch
  * plans from the current state -- persistent team sizes in both exchange
  * modes, and the streaming engine -- restores the state after each, and keeps
@@ -354,6 +391,14 @@ int fem_adjoint_forward(fem_adjoint* a, const double* u0, int32_t nsteps, int32_
  * NULL.  stats->ms is the wall time of the whole backward pass. */
 int fem_adjoint_backward(fem_adjoint* a, const double* grad_uT, double* grad_u0, double* grad_kappa_e,
                          fem_step_stats* stats);
+/* The same two calls on device pointers (FP64 arrays in device memory, for
+ * example torch tensors): no host copy.  Work that produces the inputs on
+ * another stream must be complete before the call; the outputs are complete
+ * when it returns. */
+int fem_adjoint_forward_device(fem_adjoint* a, const double* u0, int32_t nsteps, int32_t checkpoint_every,
+                               double* uT, fem_step_stats* stats);
+int fem_adjoint_backward_device(fem_adjoint* a, const double* grad_uT, double* grad_u0, double* grad_kappa_e,
+                                fem_step_stats* stats);
 
 #ifdef __cplusplus
 }

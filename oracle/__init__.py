@@ -21,6 +21,15 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   smooth_normals          closed form on flat meshes (2 x incident area along z),
                           outward normals on the sphere
   port_run_f              same steps as orc_step_f (FP32) up to CG tolerance
+  read_obj                hand-built OBJ files with known arrays (triangles,
+                          quads split either way, a pentagon, negative and
+                          v//n indices, two objects), the shipped icosphere's
+                          golden arrays and the shipped tube's counts; the
+                          quad-split tie rule of tinyobjloader itself is
+                          parity unpinned (the library is absent)
+  ply_clip, viridis       hand-computed clipping cases (one, two, three
+                          vertices above the threshold, closed and mirrored
+                          caps), Viridis endpoints
 """
 from __future__ import annotations
 
@@ -321,3 +330,176 @@ def port_run(sysf, equation, prm, nsteps, u, v=None):
     its = lib.orc_port_run_f(n, _p(K[0]), _p(K[1]), _p(K[2]), _p(M[0]), _p(M[1]), _p(M[2]),
                              _p(A[0]), _p(A[1]), _p(A[2]), equation, _p(pr), _p(u), _p(v), nsteps)
     return its, u, v
+
+
+# ---- surface input and output (Surface.cpp:57-133, 138-285) ---------------
+
+def read_obj(path):
+    """Surface::init_from_obj (src/Utils/Surface.cpp:57-133) restated:
+    xyz, tri, on_boundary and vertex normals.  Faces are triangulated as
+    tinyobjloader's triangulate option does: triangles kept, quads split along
+    the shorter diagonal (squared lengths in float32; [0,1,2],[0,2,3] when
+    |v2-v0|^2 < |v3-v1|^2, else [0,1,3],[1,2,3]), larger polygons fanned.  Each
+    object/group re-appends the whole vertex list while its indices stay
+    unshifted (Surface.cpp:77-80); vertex normals are the float32 sums of the
+    triangulated corners' normals, normalised as glm::normalize (NaN where no
+    corner contributes); boundary by the edge-count rule (:104-127)."""
+    f32 = np.float32
+    verts, norms = [], []
+    shapes = [[]]  # triangulated corners (v, n)
+
+    def fix(raw, count):
+        return raw - 1 if raw > 0 else count + raw
+
+    with open(path, "r") as fh:
+        for line in fh:
+            parts = line.split()
+            if not parts or parts[0].startswith("#"):
+                continue
+            tag = parts[0]
+            if tag in ("v", "vn"):
+                (verts if tag == "v" else norms).append([f32(float(x)) for x in parts[1:4]])
+            elif tag in ("o", "g"):
+                if shapes[-1]:
+                    shapes.append([])
+            elif tag == "f":
+                face = []
+                for tok in parts[1:]:
+                    sub = tok.split("/")
+                    v = fix(int(sub[0]), len(verts))
+                    n = fix(int(sub[2]), len(norms)) if len(sub) >= 3 and sub[2] else -1
+                    face.append((v, n))
+                if len(face) == 3:
+                    shapes[-1] += face
+                elif len(face) == 4:
+                    p = [np.asarray(verts[c[0]], dtype=f32) for c in face]
+                    a, b = p[2] - p[0], p[3] - p[1]
+                    s02 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2]
+                    s13 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2]
+                    order = (0, 1, 2, 0, 2, 3) if s02 < s13 else (0, 1, 3, 1, 2, 3)
+                    shapes[-1] += [face[q] for q in order]
+                elif len(face) > 4:
+                    for q in range(1, len(face) - 1):
+                        shapes[-1] += [face[0], face[q], face[q + 1]]
+    if len(shapes) > 1 and not shapes[-1]:
+        shapes.pop()
+    if not verts:
+        raise ValueError(f"File {path} does not contain a valid .obj mesh.")
+    if not norms:
+        raise ValueError(f"File {path} does not contain normals!")
+    V = np.asarray(verts, dtype=f32)
+    Nn = np.asarray(norms, dtype=f32)
+    xyz = np.concatenate([V] * len(shapes))
+    corners = [c for sh in shapes for c in sh]
+    tri = np.asarray([c[0] for c in corners], dtype=np.uint32).reshape(-1, 3)
+    nrm = np.zeros_like(xyz)
+    for v, n in corners:
+        if n >= 0:
+            nrm[v] = nrm[v] + Nn[n]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2]
+        inv = f32(1.0) / np.sqrt(d)
+        nrm = nrm * inv[:, None]
+    t = tri.astype(np.int64)
+    onb = np.zeros(xyz.shape[0], dtype=np.uint8)
+    if t.size:
+        a = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+        b = np.concatenate([t[:, 1], t[:, 2], t[:, 0]])
+        key = np.minimum(a, b) * np.int64(xyz.shape[0]) + np.maximum(a, b)
+        uniq, counts = np.unique(key, return_counts=True)
+        once = uniq[counts == 1]
+        onb[once // xyz.shape[0]] = 1
+        onb[once % xyz.shape[0]] = 1
+    return {"xyz": xyz, "tri": tri, "on_boundary": onb, "normals": nrm}
+
+
+# Viridis as an order-6 polynomial per channel: the reference's default colour
+# map (src/Utils/AssetStorage.cpp:141-152).
+VIRIDIS = np.array([
+    [0.274344, 0.004462, 0.331359],
+    [0.108915, 1.397291, 1.388110],
+    [-0.319631, 0.243490, 0.156419],
+    [-4.629188, -5.882803, -19.646115],
+    [6.181719, 14.388598, 57.442181],
+    [4.876952, -13.955112, -66.125783],
+    [-5.513165, 4.709245, 26.582180],
+], dtype=np.float32)
+OPEN, CLOSED, MIRRORED = 0, 1, 2  # enum class MeshType (include/Utils/Surface.hpp:25-29)
+
+
+def viridis(t):
+    """ColorMap::get_color (src/Utils/ColorMap.cpp:12-14) in float32."""
+    t = np.float32(t)
+    acc = VIRIDIS[6]
+    for k in range(5, -1, -1):
+        acc = VIRIDIS[k] + t * acc
+    return acc
+
+
+def ply_clip(xyz, tri, normals, values, vertex_extrusion, threshold, mesh_type=OPEN):
+    """Surface::export_to_ply's clipping (src/Utils/Surface.cpp:146-255) in
+    float32: positions, their values and faces.  Positions are keyed by value
+    (glm::vec3 equality: -0.0 equals 0.0) in first-use order."""
+    f = np.float32
+    V = np.asarray(xyz, dtype=f).reshape(-1, 3)
+    Nm = np.asarray(normals, dtype=f).reshape(-1, 3)
+    val = np.asarray(values, dtype=f).reshape(-1)
+    T = np.asarray(tri, dtype=np.int64).reshape(-1, 3)
+    ext, thr = f(vertex_extrusion), f(threshold)
+    positions, pvalues, faces = [], [], []
+    index = {}
+
+    def add(pos, value):
+        key = tuple(float(z) + 0.0 for z in pos)  # + 0.0 maps -0.0 to 0.0
+        if key not in index:
+            index[key] = len(positions)
+            positions.append(pos.astype(f))
+            pvalues.append(f(value))
+        return index[key]
+
+    def mix(x, y, a):
+        a = f(a)
+        return x * (f(1.0) - a) + y * a
+
+    def extruded(i):
+        return (val[i] * max(f(0.0), ext)) * Nm[i] + V[i]
+
+    def projected(i):
+        scale = (thr - val[i] * ext) if mesh_type == MIRRORED else (thr * ext)
+        return f(scale) * Nm[i] + V[i]
+
+    for t in T:
+        above = [bool(float(val[i] - thr) >= 1e-9) for i in t]
+        count = sum(above)
+        if count == 1:
+            ai = above.index(True)
+            clipped = [0, 0, 0]
+            for i in range(3):
+                if i != ai:
+                    w = (thr - val[t[i]]) / (val[t[ai]] - val[t[i]])
+                    clipped[i] = add(mix(extruded(t[i]), extruded(t[ai]), w), thr)
+                else:
+                    clipped[i] = add(extruded(t[i]), val[t[ai]])
+            faces.append(tuple(clipped))
+            if mesh_type != OPEN:
+                faces.append(tuple(clipped[i] if i != ai else add(projected(t[i]), thr) for i in range(3)))
+        elif count == 2:
+            a1 = 0 if above[0] else 1
+            a2 = 1 if (above[0] and above[1]) else 2
+            b = 3 - (a1 + a2)
+            p1, p2 = extruded(t[a1]), extruded(t[a2])
+            q1 = mix(extruded(t[b]), p1, (thr - val[t[b]]) / (val[t[a1]] - val[t[b]]))
+            q2 = mix(extruded(t[b]), p2, (thr - val[t[b]]) / (val[t[a2]] - val[t[b]]))
+            i1, i2 = add(p1, val[t[a1]]), add(p2, val[t[a2]])
+            j1, j2 = add(q1, thr), add(q2, thr)
+            faces += [(j1, i1, j2), (j2, i1, i2)]
+            if mesh_type != OPEN:
+                k1 = add(projected(t[a1]), val[t[a1]])
+                k2 = add(projected(t[a2]), val[t[a2]])
+                faces += [(j1, k1, j2), (j2, k1, k2)]
+        elif count == 3:
+            faces.append(tuple(add(extruded(i), val[i]) for i in t))
+            if mesh_type != OPEN:
+                faces.append(tuple(add(projected(i), thr) for i in t))
+    return (np.array(positions, dtype=f).reshape(-1, 3), np.array(pvalues, dtype=f),
+            np.array(faces, dtype=np.int64).reshape(-1, 3))

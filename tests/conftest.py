@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# torch first: the library then binds torch's HIP runtime (same soname), so
+# the process holds one runtime and device tensors can cross the C ABI.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "fea-visualizer_amd")):
     if p not in sys.path:

@@ -28,7 +28,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "fea-visualizer_amd"))
 
-from femhip.meshes import jittered_square, read_obj, structured_square  # noqa: E402
+from femhip.meshes import jittered_square, structured_square  # noqa: E402
+
+sys.path.insert(0, ROOT)
+from oracle import read_obj  # noqa: E402  (the reader's restatement, not the library's)
 
 REF_ICO = "/root/reference/assets/fem_meshes/icosphere.obj"
 

@@ -219,3 +219,31 @@ def test_long_row_in_one_workgroup(gpu):
     assert sol.advance(5) == _lib.FEM_OK
     _, u, _, _ = s.run(oracle.HEAT, [1e-3, 1.0], 5, u0.copy(), tol=1e-14)
     assert rel(sol.get_state_unknowns(), u) <= 1e-8
+
+
+@pytest.mark.parametrize("mesh_type", [_lib.FEM_MESH_OPEN, _lib.FEM_MESH_MIRRORED])
+def test_solver_export_ply_from_device(gpu, tmp_path, mesh_type):
+    """fem_solver_export_ply writes the solver's current surface field (v for
+    Gray-Scott) exactly as the clipping restatement does on the values
+    get_state returns."""
+    import test_surface_io as sio
+    m = meshes.icosphere(3)
+    ctx = FEMContext(m, 1)
+    ctx.equation = Equation.Reaction_Diffusion
+    ctx.parameters[Equation.Reaction_Diffusion] = ReactionDiffusionParameters(
+        0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    sol = HIPSolver(ctx, SolverOptions())
+    n = ctx.num_unknowns()
+    sol.set_state_unknowns(_lib.FEM_FIELD_U, np.ones(n))
+    sol.set_state_unknowns(_lib.FEM_FIELD_V, np.where(np.arange(n) % 9 == 0, 0.25, 0.0))
+    assert sol.advance(3) == _lib.FEM_OK
+    x = m["xyz"].astype(np.float32)
+    nrm = x / np.linalg.norm(x, axis=1, keepdims=True)
+    vals = sol.get_state(_lib.FEM_FIELD_V)
+    thr = float(np.quantile(vals, 0.6))
+    path = str(tmp_path / "solver.ply")
+    sol.export_ply(path, nrm, 2.0, thr, mesh_type)
+    head, rows, faces = sio.read_ply_exact(path)
+    pos, pval, fref = oracle.ply_clip(x, m["tri"], nrm, vals, 2.0, thr, mesh_type)
+    assert len(pos) > 0 and len(fref) > 0
+    assert sio.same(rows[:, :3], pos) and np.array_equal(faces[:, 1:], fref)

@@ -162,27 +162,35 @@ def test_adjoint_central_differences_heat_fp64(gpu):
     adj.close()
 
 
-def test_torch_autograd_rollout(gpu):
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_torch_autograd_rollout(gpu, device):
+    """Rollout through torch.autograd against the oracle's FP64 adjoint, with
+    host tensors and with device tensors (device pointers cross the ABI);
+    a second call with a changed kappa re-assembles."""
     import torch
     from femhip.autograd import Rollout
 
     m, ve, kappa = c5_like(16, 13)
     dt, T = 2e-3, 9
-    s = oracle.System(m, bc=0, velocity_e=ve, normalize=False, kappa_e=kappa)
     rng = np.random.Generator(np.random.PCG64(14))
+    s = oracle.System(m, bc=0, velocity_e=ve, normalize=False, kappa_e=kappa)
     u0 = rng.standard_normal(s.n)
     target = rng.standard_normal(s.n)
-    loss, uT, gu, gk = s.adjoint(dt, T, u0, target, tol=1e-14)
     roll = Rollout(m, AdvectionDiffusionParameters(dt, 1.0), T, checkpoint_every=4, velocity_e=ve,
                    normalize_velocity=False, options=SolverOptions(precision=_lib.FEM_PREC_FP64, tol=1e-14))
-    tu0 = torch.tensor(u0, dtype=torch.float64, requires_grad=True)
-    tk = torch.tensor(kappa, dtype=torch.float64, requires_grad=True)
-    out = roll(tu0, tk)
-    L = ((out - torch.tensor(target, dtype=torch.float64)) ** 2).sum()
-    L.backward()
-    assert abs(L.item() - loss) <= 1e-10 * loss
-    assert rel(tu0.grad.numpy(), gu) <= 1e-9
-    assert rel(tk.grad.numpy(), gk) <= 1e-9
+    for kap in (kappa, kappa * 1.5):
+        so = oracle.System(m, bc=0, velocity_e=ve, normalize=False, kappa_e=kap)
+        loss, uT, gu, gk = so.adjoint(dt, T, u0, target, tol=1e-14)
+        tu0 = torch.tensor(u0, dtype=torch.float64, device=device, requires_grad=True)
+        tk = torch.tensor(kap, dtype=torch.float64, device=device, requires_grad=True)
+        out = roll(tu0, tk)
+        assert out.device.type == device
+        L = ((out - torch.tensor(target, dtype=torch.float64, device=device)) ** 2).sum()
+        L.backward()
+        assert tu0.grad.device.type == device and tk.grad.device.type == device
+        assert abs(L.item() - loss) <= 1e-10 * loss
+        assert rel(tu0.grad.cpu().numpy(), gu) <= 1e-9
+        assert rel(tk.grad.cpu().numpy(), gk) <= 1e-9
     roll.close()
 
 

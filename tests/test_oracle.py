@@ -358,7 +358,7 @@ def test_shipped_tube_counts():
     vertices; Dirichlet N = 4,512, nnz 31,200; Neumann nnz 32,544.  (The quad
     split rule of tinyobjloader is unpinned here; the counts do not depend on
     the diagonal chosen.)"""
-    m = meshes.read_obj(TUBE)
+    m = oracle.read_obj(TUBE)
     s = oracle.System(m, bc=0)
     assert (s.nv, s.ntri, int(m["on_boundary"].sum()), s.n, s.nnz) == (4704, 9216, 192, 4512, 31200)
     assert oracle.System(m, bc=1).nnz == 32544
@@ -394,42 +394,62 @@ def test_smooth_normals_oracle_closed_forms():
     assert np.all((n_out * p).sum(axis=1) > 0)
 
 
-def test_ply_export_clipping_cases(tmp_path):
-    """Surface::export_to_ply's three clipping cases on one triangle, the
-    closed and mirrored caps, de-duplication and the file round trip."""
-    from femhip import export
+def test_ply_clip_oracle_cases():
+    """The restatement of Surface::export_to_ply's clipping on one triangle:
+    the three clipping cases, the closed cap, nothing above the threshold,
+    and Viridis's endpoints."""
     xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
     nrm = np.tile(np.float32([0, 0, 1]), (3, 1))
     tri = np.array([[0, 1, 2]])
     # all above: one triangle, positions extruded by value * extrusion
-    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 2.0, 3.0], 0.5, 0.0)
+    pos, val, f = oracle.ply_clip(xyz, tri, nrm, [1.0, 2.0, 3.0], 0.5, 0.0)
     assert f.tolist() == [[0, 1, 2]] and np.allclose(pos[:, 2], [0.5, 1.0, 1.5])
     # one above (vertex 0 at 1, others 0, threshold 0.5): the cut points sit half way
-    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 0.0, 0.0], 1.0, 0.5)
+    pos, val, f = oracle.ply_clip(xyz, tri, nrm, [1.0, 0.0, 0.0], 1.0, 0.5)
     assert len(f) == 1 and len(pos) == 3
     assert np.allclose(sorted(map(tuple, pos.tolist())), sorted([(0, 0, 1), (0.5, 0, 0.5), (0, 0.5, 0.5)]))
     assert np.allclose(sorted(val.tolist()), [0.5, 0.5, 1.0])
     # two above: a quad split in two triangles; CLOSED adds the projected cap (2 more faces)
-    pos, val, f = export.clip_surface(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5)
+    pos, val, f = oracle.ply_clip(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5)
     assert len(f) == 2 and len(pos) == 4
-    pos_c, _, f_c = export.clip_surface(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5, export.CLOSED)
+    assert f.tolist() == [[2, 0, 3], [3, 0, 1]]
+    pos_c, _, f_c = oracle.ply_clip(xyz, tri, nrm, [1.0, 1.0, 0.0], 1.0, 0.5, oracle.CLOSED)
     assert len(f_c) == 4 and len(pos_c) == 6
+    # closed cap: the kept vertices projected to threshold * extrusion above the plane
+    assert np.allclose(pos_c[4:, 2], 0.5)
     # none above: nothing
-    pos, val, f = export.clip_surface(xyz, tri, nrm, [0.0, 0.0, 0.0], 1.0, 0.5)
+    pos, val, f = oracle.ply_clip(xyz, tri, nrm, [0.0, 0.0, 0.0], 1.0, 0.5)
     assert len(f) == 0 and len(pos) == 0
-    # shared vertices are written once; file round trip and colours
-    m = meshes.structured_square(4)
-    n = np.tile(np.float32([0, 0, 1]), (m["xyz"].shape[0], 1))
-    v = np.linspace(0.0, 1.0, m["xyz"].shape[0])
-    path = str(tmp_path / "s.ply")
-    pos, val, f = export.export_ply(path, m["xyz"], m["tri"], n, v, 0.2, -1.0)
-    assert len(pos) == m["xyz"].shape[0] and len(f) == m["tri"].shape[0]
-    p2, col, f2 = export.read_ply(path)
-    assert np.allclose(p2, pos, rtol=1e-6, atol=1e-7) and np.array_equal(f2, f)
-    cm = export.polynomial_color_map(export.VIRIDIS)
-    assert np.allclose(col[0], cm(val[0]), atol=1e-6)
     # viridis endpoints: dark purple at 0, yellow at 1
-    assert cm(0.0)[2] > cm(0.0)[1] and cm(1.0)[0] > 0.9 and cm(1.0)[1] > 0.85
+    c0, c1 = oracle.viridis(0.0), oracle.viridis(1.0)
+    assert c0[2] > c0[1] and c1[0] > 0.9 and c1[1] > 0.85
+
+
+def test_read_obj_oracle_hand_built(tmp_path):
+    """The OBJ restatement on a hand-built file: a triangle with v//n
+    corners, a quad split along its shorter diagonal either way, a pentagon
+    fanned, negative indices, two objects (the second repeats the vertex
+    list), normals summed per corner and normalised, edge-count boundary."""
+    text = "\n".join([
+        "# hand-built", "o first",
+        "v 0 0 0", "v 1 0 0", "v 1 1 0", "v 0 1 0", "v 2 0 0", "v 3 0.2 0",
+        "vn 0 0 1", "vn 0 0 2",
+        "f 1//1 2//1 3//2",          # triangle; vertex 3 gets (0,0,1) + (0,0,2) below
+        "f 2//1 5//1 6//1 3//2",     # quad: |6-2|^2 = 4.04 > |3-5|^2 = 2 -> [0,1,3],[1,2,3]
+        "o second",
+        "f -6//-2 -4//-2 -3//-1",    # negative indices: vertices 1, 3, 4
+        "f 1//1 3//1 4//1 5//1 6//1",  # pentagon, fanned from vertex 1
+    ]) + "\n"
+    path = tmp_path / "hand.obj"
+    path.write_text(text)
+    m = oracle.read_obj(str(path))
+    assert m["xyz"].shape == (12, 3)          # two objects: the vertex list twice
+    assert np.array_equal(m["xyz"][:6], m["xyz"][6:])
+    assert m["tri"].tolist() == [[0, 1, 2], [1, 4, 2], [4, 5, 2], [0, 2, 3], [0, 2, 3], [0, 3, 4], [0, 4, 5]]
+    # every used vertex's normal is +z; unused copies are NaN
+    assert np.allclose(m["normals"][:6], [0, 0, 1])
+    assert np.all(np.isnan(m["normals"][6:]))
+    assert m["on_boundary"][6:].sum() == 0
 
 
 def test_gl_step_oracle_pins():
