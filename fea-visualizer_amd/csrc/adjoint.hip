@@ -170,7 +170,8 @@ int adjoint_forward(fem_adjoint* a, const double* u0, int nsteps, int every, dou
     FEM_HIP(hipMemcpyAsync(state_u(a->fwd), a->d_ckpt, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     FEM_HIP(hipStreamSynchronize(st));
   }
-  a->fwd->insts[0].hist = 0;
+  int hr = batch_reset_history(a->fwd, 0);
+  if (hr) return hr;
   InstHost& h = a->fwd->insts[0];
   h.traj = a->d_ckpt + n;
   h.traj_every = every;
@@ -252,14 +253,14 @@ int adjoint_backward(fem_adjoint* a, const double* gT, double* gu0, double* gk, 
     acc_stats(s, r);
     FEM_HIP(hipMemcpyAsync(ha.d_u, ha.d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, a->adj->stream));
     FEM_HIP(hipStreamSynchronize(a->adj->stream));
-    ha.hist = 0;
+    if ((r = batch_reset_history(a->adj, 0))) return r;
     const int nseg = (T + C - 1) / C;
     for (int j = nseg - 1; j >= 0; j--) {
       const int t0 = j * C, t1 = std::min(T, t0 + C), L = t1 - t0;
       // recompute u_{t0+1} .. u_{t1} from the checkpoint u_{t0}
       FEM_HIP(hipMemcpyAsync(hf.d_u, a->d_ckpt + (size_t)j * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
       FEM_HIP(hipStreamSynchronize(st));
-      hf.hist = 0;
+      if ((r = batch_reset_history(a->fwd, 0))) return r;
       hf.traj = a->d_U;
       hf.traj_every = 1;
       r = batch_advance(a->fwd, L, &s);

@@ -275,7 +275,8 @@ int fem_solver_state_device_ptr(fem_solver* s, int32_t field, void** dptr) {
     int rc = batch_field_ptr(s->batch, 0, field, &d, &col, &nc);
     if (rc) return rc;
     *dptr = (void*)d;
-    s->batch->insts[0].hist = 0;  // the caller may write the state: restart the predictor
+    rc = batch_reset_history(s->batch, 0);  // the caller may write the state: restart the predictor
+    if (rc) return rc;
     return FEM_OK;
   })
 }

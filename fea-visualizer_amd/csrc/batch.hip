@@ -102,6 +102,9 @@ void free_inst(InstHost& h) {
                   h.d_surf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (h.surf_done) (void)hipEventSynchronize(h.surf_done);
+  if (h.h_surf) (void)hipHostFree(h.h_surf);
+  if (h.surf_done) (void)hipEventDestroy(h.surf_done);
   fem_ctx* c = h.ctx;
   const int tr = h.transpose;
   h = InstHost();
@@ -135,6 +138,13 @@ int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
   h.d_counters = (long long*)info;
   h.d_status = (int*)(info + kInfoStatus);
   h.d_last_res = (double*)(info + kInfoLastRes);
+  h.d_hist = (int*)(info + kInfoHist);
+  {
+    hipError_t e = hipHostMalloc((void**)&h.h_surf, sizeof(float) * std::max<size_t>(1, (size_t)h.ctx->nv * k),
+                                 hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h.surf_done, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "alloc_inst: surface staging");
+  }
   const size_t vb = (size_t)n * k * sizeof(double);
   FEM_HIP(hipMemsetAsync(h.d_u, 0, vb, b->stream));
   FEM_HIP(hipMemsetAsync(h.d_v, 0, vb, b->stream));
@@ -145,6 +155,7 @@ int alloc_inst(fem_batch* b, InstHost& h, int k, int kv) {
   FEM_HIP(hipMemsetAsync(h.d_status, 0, 4 * sizeof(int), b->stream));
   FEM_HIP(hipMemsetAsync(h.d_counters, 0, 4 * sizeof(long long), b->stream));
   FEM_HIP(hipMemsetAsync(h.d_last_res, 0, kMaxK * sizeof(double), b->stream));
+  FEM_HIP(hipMemsetAsync(h.d_hist, 0, sizeof(int), b->stream));
   h.dirty = true;
   return 0;
 }
@@ -285,7 +296,8 @@ void fill_dev(fem_batch* b, InstHost& h) {
     // Newmark extrapolates the acceleration (history in h3, h4): linear or quadratic
     I.extrap = (o.predictor != FEM_PREDICTOR_LINEAR && h.d_h4) ? 2 : h.d_h3 ? 1 : 0;
   }
-  I.hist = std::min(h.hist, I.extrap);
+  I.hist = 0;  // the kernels take the held levels from hist_dev
+  I.hist_dev = h.d_hist;
 }
 
 void free_plan(fem_batch* b) {
@@ -599,7 +611,7 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
   b->insts.resize(count);
   // outcome records of every instance, then the abort and surface-change
   // flags: one copy brings all of them back after a launch
-  b->info_bytes = (size_t)count * kInfoBytes + 2 * sizeof(int);
+  b->info_bytes = (size_t)count * kInfoBytes + sizeof(int);
   {
     int rc = 0;
     b->d_info = dalloc<char>(b->info_bytes, &rc);
@@ -616,7 +628,6 @@ int batch_create(fem_ctx* const* ctxs, int count, const fem_solver_opts* opts, f
       return rc;
     }
     b->d_abort = (int*)(b->d_info + (size_t)count * kInfoBytes);
-    b->d_changed = b->d_abort + 1;
   }
   for (int i = 0; i < count; i++) {
     if (!ctxs[i]) {
@@ -713,7 +724,7 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
     }
     if (h.eq != eq || h.dt != dt || h.p1 != p1 || h.p2 != p2 || h.p3 != p3 || h.p4 != p4 || h.weighting != w) {
       h.dirty = true;
-      h.hist = 0;
+      FEM_HIP(hipMemsetAsync(h.d_hist, 0, sizeof(int), b->stream));  // the history solved another system
     }
     if (h.eq != eq) b->plan_ready = false;
     h.eq = eq;
@@ -854,8 +865,8 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
   if (n == 0) return 0;
   if (nc == h.k) {  // the field spans every column: copy straight into it
     FEM_HIP(hipMemcpyAsync(d, x, sizeof(double) * n * nc, hipMemcpyHostToDevice, b->stream));
+    FEM_HIP(hipMemsetAsync(h.d_hist, 0, sizeof(int), b->stream));  // the history no longer matches the state
     FEM_HIP(hipStreamSynchronize(b->stream));
-    h.hist = 0;  // the predictor's history no longer matches the state
     return 0;
   }
   double* tmp = dalloc<double>((size_t)n * nc, &rc);
@@ -865,10 +876,10 @@ int batch_set_state_unknowns(fem_batch* b, int inst, int field, const double* x)
     hipLaunchKernelGGL(k_put_cols, dim3(div_up(n * nc, kBlock)), dim3(kBlock), 0, b->stream, d, h.k, col, tmp, nc, n);
     e = hipGetLastError();
   }
+  if (e == hipSuccess) e = hipMemsetAsync(h.d_hist, 0, sizeof(int), b->stream);  // the history no longer matches
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "set_state_unknowns");
-  h.hist = 0;  // the predictor's history no longer matches the state
   return 0;
 }
 
@@ -900,10 +911,10 @@ int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x) {
 // whose state, rounded to float, already equals the value is left alone: a
 // caller that writes back what get_state returned (the frame loop of
 // Application.cpp:530-540 gathers and scatters every frame) keeps the FP64
-// state and the predictor history.  *changed is set when any entry was
-// written.
+// state and the predictor history.  A written entry restarts the predictor:
+// the held history levels (*hist) become 0.
 __global__ void k_surface_gather(const int* idx, int nv, const float* surf, int nc, double* state, int k, int col,
-                                 int* changed) {
+                                 int* hist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const int r = idx[i];
@@ -917,7 +928,7 @@ __global__ void k_surface_gather(const int* idx, int nv, const float* surf, int 
       any = true;
     }
   }
-  if (any) *changed = 1;
+  if (any) *hist = 0;
 }
 
 // State -> surface values, boundary vertices 0 (CPUSolver.cpp:28-37).
@@ -936,15 +947,15 @@ int batch_set_surface(fem_batch* b, int inst, int field, const float* values_nv)
   InstHost& h = b->insts[inst];
   fem_ctx* c = h.ctx;
   if (c->nv == 0) return 0;
-  FEM_HIP(hipMemcpyAsync(h.d_surf, values_nv, sizeof(float) * c->nv * nc, hipMemcpyHostToDevice, b->stream));
-  FEM_HIP(hipMemsetAsync(b->d_changed, 0, sizeof(int), b->stream));
+  // the values go through pinned staging, so the call returns without waiting
+  // for the device: the previous upload from the staging must have completed
+  FEM_HIP(hipEventSynchronize(h.surf_done));
+  std::memcpy(h.h_surf, values_nv, sizeof(float) * c->nv * nc);
+  FEM_HIP(hipMemcpyAsync(h.d_surf, h.h_surf, sizeof(float) * c->nv * nc, hipMemcpyHostToDevice, b->stream));
+  FEM_HIP(hipEventRecord(h.surf_done, b->stream));
   hipLaunchKernelGGL(k_surface_gather, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, b->stream, c->d_idx, c->nv,
-                     h.d_surf, nc, d, h.k, col, b->d_changed);
+                     h.d_surf, nc, d, h.k, col, h.d_hist);
   FEM_HIP(hipGetLastError());
-  int* hc = (int*)(b->h_info + b->insts.size() * kInfoBytes) + 1;
-  FEM_HIP(hipMemcpyAsync(hc, b->d_changed, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  FEM_HIP(hipStreamSynchronize(b->stream));
-  if (*hc) h.hist = 0;  // the predictor's history no longer matches the state
   return 0;
 }
 
@@ -959,8 +970,20 @@ int batch_get_surface(fem_batch* b, int inst, int field, float* values_nv) {
   hipLaunchKernelGGL(k_surface_scatter, dim3(div_up(c->nv, kBlock)), dim3(kBlock), 0, b->stream, c->d_idx, c->nv,
                      d, h.k, col, nc, h.d_surf);
   FEM_HIP(hipGetLastError());
+  // straight into the caller's array (the pinned staging serves uploads only:
+  // reading it back on the host is no faster)
   FEM_HIP(hipMemcpyAsync(values_nv, h.d_surf, sizeof(float) * c->nv * nc, hipMemcpyDeviceToHost, b->stream));
   FEM_HIP(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// Restart the predictor of one instance (or all, inst < 0): its history
+// levels become 0 (enqueued on the batch stream).
+int batch_reset_history(fem_batch* b, int inst) {
+  for (int i = 0; i < (int)b->insts.size(); i++) {
+    if (inst >= 0 && i != inst) continue;
+    if (b->insts[i].d_hist) FEM_HIP(hipMemsetAsync(b->insts[i].d_hist, 0, sizeof(int), b->stream));
+  }
   return 0;
 }
 
@@ -975,7 +998,7 @@ int batch_clear(fem_batch* b, int inst) {
     FEM_HIP(hipMemsetAsync(h.d_a, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_x, 0, vb, b->stream));
     FEM_HIP(hipMemsetAsync(h.d_status, 0, sizeof(int), b->stream));
-    h.hist = 0;
+    FEM_HIP(hipMemsetAsync(h.d_hist, 0, sizeof(int), b->stream));
   }
   FEM_HIP(hipStreamSynchronize(b->stream));
   return 0;
@@ -1030,9 +1053,8 @@ int read_info(fem_batch* b) {
   return 0;
 }
 
-int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
-  int rc = read_info(b);
-  if (rc) return rc;
+// Fold the host copy of the info block into the call's statistics and status.
+int parse_stats(fem_batch* b, fem_step_stats* stats, float ms) {
   long long its = 0, sw = 0, steps = 0;
   int status = 0;
   double res = 0.0;
@@ -1064,6 +1086,12 @@ int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
   if (status & (ST_UNSTABLE | ST_NONFINITE)) return FEM_DIVERGED;
   if (status & ST_NOT_CONVERGED) return FEM_NOT_CONVERGED;
   return FEM_OK;
+}
+
+int collect_stats(fem_batch* b, fem_step_stats* stats, float ms) {
+  const int rc = read_info(b);
+  if (rc) return rc;
+  return parse_stats(b, stats, ms);
 }
 
 // Abort flag of the last launch, from the host copy read_info made.
@@ -1129,18 +1157,18 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     if (rc) return rc;
   }
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
-  FEM_HIP(hipEventSynchronize(b->ev1));
+  // one wait for the launch and its outcome records (the kernel updates the
+  // predictor history levels itself)
+  rc = read_info(b);
+  if (rc) return rc;
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  rc = collect_stats(b, stats, ms);
-  if (rc == FEM_ERR_HIP || rc == FEM_ERR_OOM) return rc;
+  rc = parse_stats(b, stats, ms);
   if (info_abort(b)) {
-    for (auto& h : b->insts) h.hist = 0;
+    batch_reset_history(b, -1);
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
-  if (nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(4, h.hist + nsteps);
   return rc;
 }
 
@@ -1217,17 +1245,16 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
   FEM_HIP(hipEventRecord(b->ev0, b->stream));
   FEM_HIP(hipGraphLaunch(b->graph, b->stream));
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
-  FEM_HIP(hipEventSynchronize(b->ev1));
+  rc = read_info(b);
+  if (rc) return rc;
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  rc = collect_stats(b, stats, ms);
-  if (rc == FEM_ERR_HIP || rc == FEM_ERR_OOM) return rc;
+  rc = parse_stats(b, stats, ms);
   if (info_abort(b)) {
-    for (auto& h : b->insts) h.hist = 0;
+    batch_reset_history(b, -1);
     set_error("device barrier wait timed out; results of this call are invalid");
     return FEM_ERR_TIMEOUT;
   }
-  for (auto& h : b->insts) h.hist = std::min(4, h.hist + b->graph_nsteps);
   return rc;
 }
 
@@ -1274,13 +1301,11 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream);
   if (rc) return rc;
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
-  FEM_HIP(hipEventSynchronize(b->ev1));
+  rc = read_info(b);
+  if (rc) return rc;
   float ms = 0.f;
   FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  rc = collect_stats(b, stats, ms);
-  if (rc >= 0 && nsteps > 0)
-    for (auto& h : b->insts) h.hist = std::min(4, h.hist + nsteps);
-  return rc;
+  return parse_stats(b, stats, ms);
 }
 
 // Autotuner: run `probe` steps under a few launch plans from the same state
@@ -1295,8 +1320,17 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   rc = step_max_coresident(b->launch.k, b->launch.inner_fp32, b->launch.kr, 1, &maxb);
   if (rc) return rc;
   const bool single = b->insts.size() == 1;
+  // Rows x columns per lane of the whole co-resident grid: beyond ~64 every
+  // phase of a persistent plan walks thousands of rows per lane at the
+  // occupancy of one long-lived grid, and a probe of it runs for seconds;
+  // such batches only try the streaming engine.
+  double lanes_work = 0.0;
+  for (auto& h : b->insts) lanes_work += (double)h.ctx->n * h.k;
+  const bool persistent_hopeless = lanes_work > 64.0 * maxb * kBlock;
   std::vector<int> cands;
-  if (single) {
+  if (persistent_hopeless) {
+    // no persistent candidates
+  } else if (single) {
     const int g0 = b->team_sizes[0];
     for (double f : {0.25, 0.5, 0.75, 1.0, 1.5, 2.0}) {
       const int g = std::max(1, std::min(maxb, (int)std::lround(g0 * f)));
@@ -1314,26 +1348,34 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   }
   // snapshot the state
   std::vector<double*> snap;
+  size_t want = 0;
   for (auto& h : b->insts) {
     const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
     for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4}) {
       if (!src) continue;
+      want++;
+      if (rc) continue;
       double* d = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
-      if (!d) break;
-      FEM_HIP(hipMemcpyAsync(d, src, vb, hipMemcpyDeviceToDevice, b->stream));
+      if (!d) continue;
       snap.push_back(d);
+      const hipError_t e = hipMemcpyAsync(d, src, vb, hipMemcpyDeviceToDevice, b->stream);
+      if (e != hipSuccess) rc = hip_fail(e, "autotune: snapshot");
     }
   }
-  std::vector<int> hist0;
-  for (auto& h : b->insts) hist0.push_back(h.hist);
+  // the info block holds each instance's predictor history levels
+  char* info0 = rc ? nullptr : dalloc<char>(b->info_bytes, &rc);
+  if (info0) FEM_HIP(hipMemcpyAsync(info0, b->d_info, b->info_bytes, hipMemcpyDeviceToDevice, b->stream));
+  // restore is only ever run on a complete snapshot
+  const bool snap_ok = !rc && info0 && snap.size() == want;
   auto restore = [&]() -> int {
-    size_t q = 0, i = 0;
+    if (!snap_ok) return FEM_ERR_OOM;
+    size_t q = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
       for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4})
         if (dst) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
-      h.hist = hist0[i++];
     }
+    FEM_HIP(hipMemcpyAsync(b->d_info, info0, b->info_bytes, hipMemcpyDeviceToDevice, b->stream));
     return 0;
   };
   const int saved_team = b->opts.team_size, saved_budget = b->grid_budget, saved_mode = b->mode_override;
@@ -1398,10 +1440,12 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     b->opts.team_size = team;
     b->opts.engine = FEM_ENGINE_PERSISTENT;
   }
-  const int rr = restore();
+  const int rr = snap_ok ? restore() : 0;
   if (!rc) rc = rr;
-  FEM_HIP(hipStreamSynchronize(b->stream));
+  const hipError_t se = hipStreamSynchronize(b->stream);
   for (double* d : snap) (void)hipFree(d);
+  if (info0) (void)hipFree(info0);
+  if (se != hipSuccess && !rc) rc = hip_fail(se, "autotune");
   if (single) b->opts.team_size = rc ? saved_team : best_c;
   else b->grid_budget = rc ? saved_budget : best_c;
   b->mode_override = rc ? saved_mode : best_m;

@@ -94,8 +94,11 @@ struct Inst {
   int extrap;            // predictor order: 0 x_n, 1 2 x_n - x_{n-1} (x_{n-1} kept in `a`),
                          //   2 3 x_n - 3 x_{n-1} + x_{n-2} (x_{n-2} kept in `v`),
                          //   3 4 x_n - 6 x_{n-1} + 4 x_{n-2} - x_{n-3} (x_{n-3} in `h3`)
-  int hist;              // history levels held at launch start (0..extrap)
-  int pad2_;
+  int hist;              // history levels used at launch start (0..extrap)
+  int hist_raw;          // history levels held at launch start, before the cap by extrap
+  int* hist_dev;         // optional: the held levels in device memory; the stepping kernels
+                         //   read it at launch start and store min(4, levels + nsteps) at the end,
+                         //   and a state write that changes the state stores 0
   double* h3;            // [n][k] x_{n-3} (cubic and quartic predictors)
   double* h4;            // [n][k] x_{n-4} (quartic predictor: 5 x_n - 10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4})
   double* xnext;         // [n][k] next initial guess formed with the update (persistent kernel)

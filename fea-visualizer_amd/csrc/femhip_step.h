@@ -59,8 +59,6 @@ struct InstHost {
   int eq_override = -1;      // >= 0: equation id the next launches run (EQ_SOLVE)
   double* traj = nullptr;    // trajectory record (see Inst::traj)
   int traj_every = 0;
-  int hist = 0;              // predictor history levels held (x_{n-1} in `a`, x_{n-2} in `v`,
-                             // x_{n-3} in d_h3)
   double* d_h3 = nullptr;    // cubic / quartic predictor history
   double* d_h4 = nullptr;
   double* d_xnext = nullptr; // Inst::xnext, Inst::hpre
@@ -80,16 +78,20 @@ struct InstHost {
   int* d_status = nullptr;         // these three point into the batch's info block
   long long* d_counters = nullptr;
   double* d_last_res = nullptr;
+  int* d_hist = nullptr;     // predictor history levels held (Inst::hist_dev), in the info block
   float* d_surf = nullptr;   // surface-space staging of set_state / get_state [nv][k]
+  float* h_surf = nullptr;   // pinned host staging of set_state's values [nv][k]
+  hipEvent_t surf_done = nullptr;  // the last upload from h_surf has completed
   Inst dev{};                // mirror of the device descriptor
 };
 
-// Per-instance outcome record in the batch's info block: counters [0, 32),
-// status bits [32, 48), final residuals [48, 48 + 8 kMaxK).  After the
-// instance records: the launch abort flag and the surface-change flag.
+// Per-instance record in the batch's info block: counters [0, 32), status
+// bits [32, 48), final residuals [48, 48 + 8 kMaxK), predictor history levels
+// held (Inst::hist_dev) at 176.  After the instance records: the launch abort
+// flag.
 constexpr size_t kInfoBytes = 256;
-constexpr size_t kInfoStatus = 32, kInfoLastRes = 48;
-static_assert(kInfoLastRes + 8 * kMaxK <= kInfoBytes, "info record");
+constexpr size_t kInfoStatus = 32, kInfoLastRes = 48, kInfoHist = 176;
+static_assert(kInfoLastRes + 8 * kMaxK <= kInfoHist, "info record");
 
 }  // namespace femhip
 
@@ -110,8 +112,7 @@ struct fem_batch {
   double* d_scal = nullptr;
   int grid = 0;
   int* d_abort = nullptr;       // in the info block
-  int* d_changed = nullptr;     // in the info block: set_state wrote a new value
-  char* d_info = nullptr;       // [ninst][kInfoBytes] + abort flag + change flag
+  char* d_info = nullptr;       // [ninst][kInfoBytes] + abort flag
   char* h_info = nullptr;       // pinned host copy
   size_t info_bytes = 0;
   std::vector<femhip::Inst> uploaded;  // descriptors last copied to the plan's d_insts
@@ -151,6 +152,7 @@ int batch_get_state_unknowns(fem_batch* b, int inst, int field, double* x);
 int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_clear(fem_batch* b, int inst);
 int batch_set_surface(fem_batch* b, int inst, int field, const float* values_nv);
+int batch_reset_history(fem_batch* b, int inst);
 int batch_get_surface(fem_batch* b, int inst, int field, float* values_nv);
 int batch_field_ptr(fem_batch* b, int inst, int field, double** ptr, int* col, int* k);
 int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats);

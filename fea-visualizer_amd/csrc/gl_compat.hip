@@ -307,7 +307,8 @@ int gl_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
       status_all |= flag;
       rr_max = std::max(rr_max, (double)rr);
       its_all += its;
-      h.hist = 0;
+      // no extrapolation history is kept here
+      if ((rc = batch_reset_history(b, (int)(&h - b->insts.data())))) break;
     }
     (void)hipFree(slab);
     if (rc) return rc;

@@ -380,6 +380,9 @@ int plain_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
       FEM_HIP(hipGetLastError());
     }
   }
+  // this engine keeps no extrapolation history: a later launch restarts the predictor
+  rc = batch_reset_history(b, -1);
+  if (rc) return rc;
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
   FEM_HIP(hipEventSynchronize(b->ev1));
   float ms = 0.f;

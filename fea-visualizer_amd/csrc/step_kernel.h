@@ -1602,6 +1602,7 @@ __device__ bool run_instance(const Inst& I, const Team& T, int rank, unsigned& e
     I.counters[1] += sweeps_total;
     I.counters[2] += nsteps;
     if (status) atomicOr(I.status, status);
+    if (I.hist_dev) *I.hist_dev = min(4, I.hist_raw + nsteps);
   }
   return true;
 }
@@ -1631,6 +1632,15 @@ __global__ void __launch_bounds__(kBlock) k_step(const Team* __restrict__ teams,
     __syncthreads();  // the previous instance's readers are done
     const int* src = (const int*)(insts + team_inst[qi]);
     for (int w = threadIdx.x; w < (int)(sizeof(Inst) / sizeof(int)); w += kBlock) ((int*)&sI)[w] = src[w];
+    __syncthreads();
+    // predictor history held in device memory: every workgroup of the team
+    // reads it here, before the instance's first barrier; rank 0 stores the
+    // new count after the last one (run_instance)
+    if (threadIdx.x == 0 && sI.hist_dev) {
+      const int held = *sI.hist_dev;
+      sI.hist_raw = held;
+      sI.hist = min(held, sI.extrap);
+    }
     __syncthreads();
     if (sI.n <= 0) continue;
     if (!run_instance<K, TI, KR, C>(sI, T, rank, e, sh, RC, nsteps)) return;

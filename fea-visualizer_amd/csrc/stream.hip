@@ -441,6 +441,22 @@ __global__ void k_step_end(SState* ss, const Inst* insts, int ninst, int k) {
   I.counters[2] += 1;
 }
 
+// Predictor history levels of a run: the held count is read once into the
+// run's descriptor copy (every phase launch reads that), and the new count is
+// stored after the run.
+__global__ void k_hist_load(Inst* insts, int ninst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst || !insts[i].hist_dev) return;
+  const int held = *insts[i].hist_dev;
+  insts[i].hist_raw = held;
+  insts[i].hist = min(held, insts[i].extrap);
+}
+__global__ void k_hist_store(const Inst* insts, int ninst, int nsteps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst || !insts[i].hist_dev) return;
+  *insts[i].hist_dev = min(4, insts[i].hist_raw + nsteps);
+}
+
 __global__ void k_sinit(SState* ss, const Inst* insts, int ninst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
@@ -755,6 +771,8 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   a.stage_rows = p->stage_rows;
   a.stage_nnz = p->stage_nnz;
   a.partials = p->d_partials;
+  hipLaunchKernelGGL(k_hist_load, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, const_cast<Inst*>(d_insts),
+                     p->ninst);
   hipLaunchKernelGGL(k_sinit, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts, p->ninst);
   FEM_HIP(hipGetLastError());
   int rc = 0;
@@ -845,6 +863,8 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     FEM_HIP(hipGetLastError());
   }
 #undef STREAM_PH
+  hipLaunchKernelGGL(k_hist_store, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, d_insts, p->ninst, nsteps);
+  FEM_HIP(hipGetLastError());
   return 0;
 }
 
