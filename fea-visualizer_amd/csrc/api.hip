@@ -399,6 +399,16 @@ int fem_batch_plan_info(fem_batch* b, fem_plan_info* info) {
   })
 }
 
+int fem_batch_set_wait_limit(fem_batch* b, double seconds) {
+  FEM_GUARD({
+    if (!b) {
+      set_error("null batch");
+      return FEM_ERR_INVALID;
+    }
+    return batch_set_wait_limit(b, seconds);
+  })
+}
+
 int fem_batch_sync(fem_batch* b) {
   if (!b) return FEM_ERR_INVALID;
   FEM_HIP(hipStreamSynchronize(b->stream));

@@ -522,6 +522,7 @@ int build_plan(fem_batch* b) {
     ht[t].slots = b->d_slots + poff;
     ht[t].scal = b->d_scal + (size_t)t * kSlotBufs * kMaxRed;
     ht[t].abort_flag = b->d_abort;
+    ht[t].wait_ticks = b->wait_ticks;
     // every workgroup reading every partial costs size^2 * nvals loads per
     // barrier; beyond ~16 KB per workgroup rank 0 reduces and publishes
     ht[t].allpoll = b->poll_override == 1   ? 1
@@ -1121,6 +1122,16 @@ int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats) {
   stats->residual = res;
   stats->ms = 0.0;  // device time is per call, not per instance
   stats->status = st[0];
+  return 0;
+}
+
+int batch_set_wait_limit(fem_batch* b, double seconds) {
+  if (!(seconds >= 0.0) || seconds > 3600.0) {
+    set_error("wait limit must be within [0, 3600] seconds");
+    return FEM_ERR_INVALID;
+  }
+  b->wait_ticks = (long long)(seconds * 1e8);
+  b->plan_ready = false;  // the teams carry the limit
   return 0;
 }
 

@@ -118,7 +118,12 @@ struct Team {
   Slot* slots;               // [size]
   double* scal;             // [kSlotBufs][kMaxRed] tagged sums published by rank 0
   int* abort_flag;
+  long long wait_ticks;      // longest barrier wait (wall_clock64 ticks, 100 MHz) before the launch
+                             // aborts; 0: abort at the first wait not already satisfied
 };
+
+// Default longest barrier wait: 4 s (a correct run never waits that long).
+constexpr long long kDefaultWaitTicks = 400000000LL;
 
 // ---------------------------------------------------------------------------
 // Host helpers

@@ -117,6 +117,7 @@ struct fem_batch {
   size_t info_bytes = 0;
   std::vector<femhip::Inst> uploaded;  // descriptors last copied to the plan's d_insts
   int nteams = 0;
+  long long wait_ticks = femhip::kDefaultWaitTicks;  // longest barrier wait of the persistent kernel
   int grid_budget = 0;          // multi-instance plans: workgroups to spread over (0 = all co-resident)
   int mode_override = 0;        // exchange mode forced by the autotuner (0 = by the row-cache rule)
   int poll_override = 0;        // barrier reduction forced by the autotuner: 1 every workgroup sums all
@@ -160,6 +161,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms);
 int batch_graph_capture(fem_batch* b, int nsteps);
 int batch_instance_stats(fem_batch* b, int inst, fem_step_stats* stats);
 int batch_plan_info(fem_batch* b, fem_plan_info* info);
+int batch_set_wait_limit(fem_batch* b, double seconds);
 int batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats);
 int batch_surface_values(fem_batch* b, int inst, float* d_out);

@@ -226,6 +226,7 @@ int linear_solve_noalloc(fem_ctx* c, const double coef[3], const fem_solver_opts
   T.slots = slots;
   T.scal = scal;
   T.abort_flag = abort_flag;
+  T.wait_ticks = kDefaultWaitTicks;
 
   hipStream_t st = c->stream;
   const auto t0 = std::chrono::steady_clock::now();

@@ -61,9 +61,9 @@ namespace femhip {
 
 namespace {
 
-// wall_clock64 ticks at 100 MHz: a barrier wait longer than 4 s aborts the
-// launch (it cannot happen in a correct run; it bounds every wait).
-constexpr long long kSpinTimeoutTicks = 400000000LL;
+// Every barrier wait is bounded by the team's wait_ticks (wall_clock64 ticks
+// at 100 MHz; 4 s unless the caller set another limit): a longer wait aborts
+// the launch.  It cannot happen in a correct run.
 
 struct Shared {
   double wred[kBlock / kWave][kMaxRed];
@@ -197,8 +197,12 @@ __device__ double wait_value(const Team& T, const double* p, unsigned e, int* to
   while (true) {
     const double v = ld<kCoherent>(p);
     if (has_tag(v, e)) return v;
+    if (T.wait_ticks <= 0) {  // zero limit: no waiting at all
+      *to = 1;
+      return 0.0;
+    }
     __builtin_amdgcn_s_sleep(1);
-    if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+    if ((++spins & 63) == 0 && (aborted(T) || wall_clock64() - t0 > T.wait_ticks)) {
       *to = 1;
       return 0.0;
     }

@@ -213,6 +213,7 @@ SIGNATURES = [
     ("fem_batch_graph_capture", C.c_int, [P, I32]),
     ("fem_batch_graph_launch", C.c_int, [P, C.POINTER(fem_step_stats)]),
     ("fem_batch_plan_info", C.c_int, [P, C.POINTER(fem_plan_info)]),
+    ("fem_batch_set_wait_limit", C.c_int, [P, C.c_double]),
     ("fem_batch_autotune", C.c_int, [P, I32, PI, PD]),
     ("fem_operator_create", C.c_int, [P, PD, I32, I32, C.POINTER(P)]),
     ("fem_operator_destroy", C.c_int, [P]),

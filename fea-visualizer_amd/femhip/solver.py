@@ -446,6 +446,11 @@ class Batch:
         """Replay the recorded advance; returns the step status as advance()."""
         return L.check(self._lib.fem_batch_graph_launch(self.handle, C.byref(self.last_stats)))
 
+    def set_wait_limit(self, seconds: float):
+        """Longest barrier wait of the persistent kernel before the launch
+        aborts with FEM_ERR_TIMEOUT (default 4 s; 0: no waiting at all)."""
+        L.check(self._lib.fem_batch_set_wait_limit(self.handle, float(seconds)))
+
     def plan_info(self):
         """Persistent-engine launch plan: grid, teams, exchange mode and the
         co-resident limit of the launched variant (fem_plan_info)."""

@@ -337,6 +337,12 @@ typedef struct fem_plan_info {
   int32_t coresident;
 } fem_plan_info;
 int fem_batch_plan_info(fem_batch* b, fem_plan_info* info);
+/* Watchdog of the persistent kernel's barriers (no reference counterpart):
+ * the longest a workgroup waits at a barrier before the launch aborts and
+ * the call returns FEM_ERR_TIMEOUT with the state invalid.  Default 4 s; 0
+ * aborts at the first barrier wait that is not already satisfied.  Takes
+ * effect at the next launch plan (a captured graph must be captured again). */
+int fem_batch_set_wait_limit(fem_batch* b, double seconds);
 /* Autotuner over the number of workgroups the batch plan spreads over
  * (*chosen; 0 when the streaming engine won); state restored as for
  * fem_solver_autotune. */

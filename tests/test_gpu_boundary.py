@@ -247,3 +247,37 @@ def test_solver_export_ply_from_device(gpu, tmp_path, mesh_type):
     pos, pval, fref = oracle.ply_clip(x, m["tri"], nrm, vals, 2.0, thr, mesh_type)
     assert len(pos) > 0 and len(fref) > 0
     assert sio.same(rows[:, :3], pos) and np.array_equal(faces[:, 1:], fref)
+
+
+def test_barrier_timeout_path_forced_from_host(gpu):
+    """The abort path of the persistent kernel's bounded barrier waits,
+    forced from the host with a zero wait limit: every workgroup leaves at
+    its first unsatisfied wait, the call reports FEM_ERR_TIMEOUT and the
+    batch recovers with the default limit (state written again) to within
+    1e-8 of the oracle."""
+    m = meshes.icosphere(4)
+    s = oracle.System(m, bc=1)
+    ctx = FEMContext(m, 1)
+    prm = ReactionDiffusionParameters(0.5, 2e-4, 1e-4, 0.037, 0.06, _lib.FEM_REACTION_LUMPED)
+    b = Batch([ctx], SolverOptions(exchange=_lib.FEM_EXCHANGE_COHERENT))
+    b.set_equation(0, prm)
+    u = np.ones(s.n)
+    v = np.where(np.arange(s.n) % 13 == 0, 0.25, 0.0)
+    b.set_state_unknowns(0, _lib.FEM_FIELD_U, u)
+    b.set_state_unknowns(0, _lib.FEM_FIELD_V, v)
+    assert b.plan_info().teams == 1 and b.plan_info().grid > 1
+    b.set_wait_limit(0.0)
+    print("case forced barrier timeout", flush=True)
+    with pytest.raises(_lib.FemError) as err:
+        b.advance(3)
+    assert err.value.code == _lib.FEM_ERR_TIMEOUT
+    b.set_wait_limit(4.0)
+    b.set_state_unknowns(0, _lib.FEM_FIELD_U, u)
+    b.set_state_unknowns(0, _lib.FEM_FIELD_V, v)
+    assert b.advance(3) == _lib.FEM_OK
+    _, ur, vr, _ = s.run(oracle.GRAYSCOTT, [0.5, 2e-4, 1e-4, 0.037, 0.06, 1], 3, u.copy(), v.copy(), tol=1e-14)
+    assert rel(b.get_state_unknowns(0, _lib.FEM_FIELD_U)[: s.n], ur) <= 1e-8
+    assert rel(b.get_state_unknowns(0, _lib.FEM_FIELD_V)[: s.n], vr) <= 1e-8
+    with pytest.raises(_lib.FemError):
+        b.set_wait_limit(-1.0)
+    b.close()
