@@ -72,18 +72,21 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
 PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3, "quartic": 4}  # FEM_PREDICTOR_*
 
 
-def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False, fused=False):
+def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False, fused=False, stream=False):
     """Algorithmic HBM bytes of a step sequence (int32 indices, inner vectors
     and values of ti bytes, FP64 refinement), every vector element and matrix
     entry touched once per phase (cache-perfect gathers).  A pipelined-CG
-    iteration moves 11 vectors and the matrix once; a BiCGSTAB iteration 17
-    vectors (p: r, p_old, v_old, rh read, p, v written; s: r, v read, s, t
-    written; x: xi, p, s, t, rh read, xi, r written) and the matrix twice.
+    iteration moves 11 vectors and the matrix once; a BiCGSTAB iteration the
+    matrix twice and 17 vectors on the persistent kernel (p: r, p_old, v_old,
+    rh read, p, v written; s: r, v read, s, t written; x: xi, p, s, t, rh
+    read, xi, r written), 15 on the streaming engine's multi-column layout
+    (stream: p read, v written; s: r, v, rh read, s, t written; x: xi, p, s,
+    t, v read, xi, r, p written).
     fused: first-order equations on the persistent kernel, whose residual
     pass also reads M and the history combination and writes M x and S x0,
     so the update + right-hand side phase reads no matrix."""
     if bicgstab:
-        it = 2 * nnz * (4 + ti * kv) + n * (17 * k * ti + ti * kv)
+        it = 2 * nnz * (4 + ti * kv) + n * ((15 if stream else 17) * k * ti + ti * kv)
     else:
         it = nnz * (4 + ti * kv) + n * (11 * k * ti + ti * kv)     # one Krylov iteration
     sw = nnz * (4 + 8 * kv) + n * (3 * k * 8 + k * ti + ti * kv)  # FP64 residual
@@ -510,7 +513,8 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0):
     alg = 0.0
     for i, c in enumerate(ctxs):
         si = b.instance_stats(i)
-        alg += step_bytes(c.num_unknowns(), c.info().nnz, 16, 1, si.iterations, si.sweeps, steps, bicgstab=True)
+        alg += step_bytes(c.num_unknowns(), c.info().nnz, 16, 1, si.iterations, si.sweeps, steps, bicgstab=True,
+                          stream=engine == "stream")
     ach = alg / (st.ms * 1e-3) / 1e9
     out = {"value": dof * steps / wall, "unit": "DOF·steps/s", "steps": steps, "warmup": warmup,
            "ms_per_step": 1e3 * wall / steps, "status": int(status),

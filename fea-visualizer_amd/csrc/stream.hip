@@ -33,6 +33,7 @@ struct SState {
   double gamma[kMaxK], alpha[kMaxK], beta[kMaxK], omega[kMaxK];
   double thr[kMaxK], ithr[kMaxK], bb[kMaxK];
   int active[kMaxK], reset[kMaxK];
+  int bbad[kMaxK];  // BiCGSTAB, multi-column layout: beta of column j broke down (phase S)
   int sweep_done, solve_done, redo, bkd;
   int it_s, its, sweeps, status;
   long long its_total, sweeps_total;
@@ -42,12 +43,19 @@ enum Phase : int {
   PH_RHS = 0, PH_RES, PH_PCG_INIT, PH_PCG_ITER, PH_CG_A, PH_CG_B, PH_BI_P, PH_BI_S, PH_BI_X, PH_UPD
 };
 
-template <int PH>
+// Dot products a phase sums per column.  BiCGSTAB on the multi-column layout
+// (stream_vec.h) sums rho's two parts in phase S and only <r,r> in phase X.
+template <int K, int PH>
 constexpr int nsums() {
-  return PH == PH_RHS || PH == PH_PCG_ITER || PH == PH_BI_S ? 3
-         : PH == PH_RES || PH == PH_PCG_INIT || PH == PH_CG_B || PH == PH_BI_X ? 2
+  return PH == PH_BI_S ? (vec_layout<K>() ? 5 : 3)
+         : PH == PH_BI_X ? (vec_layout<K>() ? 1 : 2)
+         : PH == PH_RHS || PH == PH_PCG_ITER ? 3
+         : PH == PH_RES || PH == PH_PCG_INIT || PH == PH_CG_B ? 2
          : PH == PH_UPD ? 0 : 1;
 }
+
+// Partial sums per row block and column: the most any phase sums.
+constexpr int kPartW = 5;
 
 struct SArgs {
   const Inst* insts;
@@ -59,7 +67,7 @@ struct SArgs {
   int step;             // step index inside the launch sequence (trajectory records)
   int pass;             // FP64 residual: 0 normal, 1 redo with the correction dropped
   int more;             // update: another step follows in this launch (it reads xnext)
-  double* partials;     // [nblocks][3 K]
+  double* partials;     // [nblocks][kPartW K]
   const int* act_blk;   // launched block -> row block (null: every row block, in order)
   const int* act_inst;  // reduction block -> instance (null: every instance)
   const int* act_cnt;   // {instances, blocks} of the lists (device); the grid may be larger
@@ -83,7 +91,7 @@ constexpr bool staged_phase() {
 // One column per lane (the persistent kernel's phase bodies).
 template <int K, typename TI, int PH>
 __device__ void run_scalar(const Inst& I, const SArgs& a, Shared& sh, int b, int row_lo, int row_hi) {
-  constexpr int NS = nsums<PH>();
+  constexpr int NS = nsums<K, PH>();
   const bool first = a.first != 0;
   const int par = a.par;
   double acc[NS > 0 ? NS : 1];
@@ -122,7 +130,7 @@ __device__ void run_scalar(const Inst& I, const SArgs& a, Shared& sh, int b, int
   if constexpr (NS > 0) {
     block_partials<K, NS, kPlain>(acc, sh, sh.red, 0);
     __syncthreads();
-    if (threadIdx.x < NS * K) a.partials[(size_t)b * 3 * K + threadIdx.x] = sh.red[threadIdx.x];
+    if (threadIdx.x < NS * K) a.partials[(size_t)b * kPartW * K + threadIdx.x] = sh.red[threadIdx.x];
   }
 }
 
@@ -131,7 +139,7 @@ __device__ void run_scalar(const Inst& I, const SArgs& a, Shared& sh, int b, int
 template <int K, int V, typename TI, int PH>
 __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int row_lo, int row_hi,
                            const Stage<TI>& g) {
-  constexpr int NS = nsums<PH>();
+  constexpr int NS = nsums<K, PH>();
   const bool first = a.first != 0;
   const int par = a.par;
   vec<double, V> acc[NS];
@@ -154,21 +162,25 @@ __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int
   } else if constexpr (PH == PH_CG_B) {
     TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
     vphase_cg_b<K, V, TI>(I, row_lo, row_hi, (TI*)I.r, pb[par ^ 1], sh, acc);
-  } else if constexpr (PH == PH_BI_P) {
-    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
-    TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
-    vphase_bi_p<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, pb[par], pb[par ^ 1], vb[par], vb[par ^ 1], first, sh,
-                          acc);
+  } else if constexpr (PH == PH_BI_P) {  // BiCGSTAB: one p and one v buffer (stream_vec.h)
+    vphase_bi_p<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, (TI*)I.p0, (TI*)I.v0, first, acc);
   } else if constexpr (PH == PH_BI_S) {
-    TI* vb[2] = {(TI*)I.v0, (TI*)I.v1};
-    vphase_bi_s<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, vb[par ^ 1], sh, acc);
+    vphase_bi_s<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, (const TI*)I.v0, sh, acc);
   } else {
-    TI* pb[2] = {(TI*)I.p0, (TI*)I.p1b};
-    vphase_bi_x<K, V, TI>(I, row_lo, row_hi, (TI*)I.r, pb[par ^ 1], sh, acc);
+    vphase_bi_x<K, V, TI>(I, row_lo, row_hi, (TI*)I.r, (TI*)I.p0, (const TI*)I.v0, sh, acc);
   }
-  vblock_partials<K, V, NS>(acc, sh);
+  // block sums in rounds of at most kMaxSlots values per column
+  constexpr int N0 = NS < kMaxSlots ? NS : kMaxSlots;
+  vblock_partials<K, V, N0>(acc, sh);
   __syncthreads();
-  if (threadIdx.x < NS * K) a.partials[(size_t)b * 3 * K + threadIdx.x] = sh.red[threadIdx.x];
+  if (threadIdx.x < N0 * K) a.partials[(size_t)b * kPartW * K + threadIdx.x] = sh.red[threadIdx.x];
+  if constexpr (NS > N0) {
+    static_assert(NS - N0 <= kMaxSlots, "two rounds of block sums");
+    __syncthreads();
+    vblock_partials<K, V, NS - N0>(acc + N0, sh);
+    __syncthreads();
+    if (threadIdx.x < (NS - N0) * K) a.partials[(size_t)b * kPartW * K + N0 * K + threadIdx.x] = sh.red[threadIdx.x];
+  }
 }
 
 template <int K, typename TI, int PH>
@@ -253,7 +265,7 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
 #pragma unroll
       for (int u = 0; u < kR; u++) {
         const int w = w0 + u * chunks;
-        v[u] = (w < b1) ? a.partials[(size_t)w * 3 * K + q] : 0.0;
+        v[u] = (w < b1) ? a.partials[(size_t)w * kPartW * K + q] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kR; u++) s += v[u];
@@ -274,7 +286,7 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
 template <int K, int PH>
 __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_blk, int ninst, int fp32) {
   __shared__ double scratch[kRedBlock];
-  __shared__ double red[3 * kMaxK];
+  __shared__ double red[kPartW * kMaxK];
   if ((int)blockIdx.x >= ninst || (a.act_cnt && (int)blockIdx.x >= a.act_cnt[0])) return;
   const int i = a.act_inst ? a.act_inst[blockIdx.x] : blockIdx.x;
   const Inst& I = a.insts[i];
@@ -284,7 +296,7 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
   } else if constexpr (PH != PH_RHS) {
     if (S.sweep_done) return;
   }
-  sum_blocks<K, nsums<PH>()>(a, inst_blk, i, red, scratch);
+  sum_blocks<K, nsums<K, PH>()>(a, inst_blk, i, red, scratch);
   if (threadIdx.x != 0) return;
   if constexpr (PH == PH_RHS) {
     int any = 0;
@@ -385,7 +397,29 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
       const bool fine = fabs(om) < 1e30;
       S.omega[j] = fine ? om : 0.0;
       if (!fine) S.bkd = 1;
+      if constexpr (vec_layout<K>()) {
+        // next rho = <rh, s - omega t>, and beta for the x phase's new p;
+        // a breakdown stops the sweep only if the column goes on (phase X)
+        const double w = S.omega[j], rho = S.gamma[j];
+        const double rho_new = red[3 * K + j] - w * red[4 * K + j];
+        const double be = (w == 0.0 || rho == 0.0) ? 0.0 : (rho_new / rho) * (S.alpha[j] / w);
+        const bool bad = be == 0.0 || !(fabs(be) < 1e30);
+        S.bbad[j] = bad;
+        S.beta[j] = bad ? 0.0 : be;
+        S.gamma[j] = rho_new;
+      }
     }
+  } else if constexpr (PH == PH_BI_X && vec_layout<K>()) {
+    S.it_s++;
+    S.its++;
+    int alldone = 1, brk = S.bkd;
+    for (int j = 0; j < K; j++) {
+      const double rr = red[j];
+      const bool done = !S.active[j] || rr < S.ithr[j] || !(rr == rr);
+      alldone &= done;
+      if (!done && S.bbad[j]) brk = 1;
+    }
+    S.sweep_done = alldone || brk || S.it_s >= I.max_inner || S.its >= I.max_iters;
   } else if constexpr (PH == PH_BI_X) {
     S.it_s++;
     S.its++;
@@ -666,7 +700,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->d_blk_row = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_blk_end = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_inst_blk = dalloc<int>(p->ninst + 1, &rc);
-  p->d_partials = dalloc<double>((size_t)std::max(1, p->nblocks) * 3 * k, &rc);
+  p->d_partials = dalloc<double>((size_t)std::max(1, p->nblocks) * kPartW * k, &rc);
   p->d_act_inst = dalloc<int>(p->ninst, &rc);
   p->d_act_base = dalloc<int>(p->ninst, &rc);
   p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);

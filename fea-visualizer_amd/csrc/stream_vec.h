@@ -8,7 +8,8 @@
 // Per element the arithmetic is the persistent kernel's (phase_* in
 // step_kernel.h): the same expressions in the same precision and the same
 // ascending summation over each row.  Only the order in which the per-lane
-// partial dot products are combined differs.  The matrix and preconditioner
+// partial dot products are combined differs, and BiCGSTAB's phases are cut
+// differently (vphase_bi_p below).  The matrix and preconditioner
 // are shared by all columns (one value column), which is the case for every
 // instance with one or with more than two right-hand sides.
 #pragma once
@@ -111,6 +112,12 @@ __device__ inline A vgather(int lo, int hi, A zero, F f) {
 template <int V>
 constexpr int vunroll() {
   return V >= 4 ? 2 : V == 2 ? 4 : 8;
+}
+// BiCGSTAB's gathers (one and two vectors per neighbour): twice the entries
+// in flight at four columns per lane measured faster on 7-point rows.
+template <int V>
+constexpr int vunroll_bi() {
+  return V >= 4 ? 4 : V == 2 ? 4 : 8;
 }
 
 // Lane layout: G = K / V lanes per row; lane columns j0 .. j0 + V - 1 with
@@ -436,101 +443,119 @@ __device__ void vphase_cg_b(const Inst& I, int row_lo, int row_hi, TI* r, const 
   }
 }
 
-// BiCGSTAB (right Jacobi preconditioning) phase P:
-// p = r + beta (p_old - omega v_old); v = S D^-1 p; <rh, v>.
+// BiCGSTAB (right Jacobi preconditioning) in three phases, arranged so that
+// the gathering phases read few vectors per neighbour.  Rho of the next
+// iteration, <rh, s - omega t> = <rh, s> - omega <rh, t>, comes out of the s
+// phase's dot products, so beta is known before the x phase, which then also
+// forms the next search direction p = r + beta (p - omega v) on its own rows:
+//   P: v = S D^-1 p; <rh, v>          (first iteration: p = rh = r, x += xi)
+//   S: s = r - alpha v; t = S D^-1 s; <t,s>, <t,t>, <s,s>, <rh,s>, <rh,t>
+//   X: xi += D^-1 (alpha p + omega s); r = s - omega t;
+//      p = r + beta (p - omega v); <r,r>
+// Phase P gathers one vector and S two (the persistent kernel's phase P
+// rebuilds p from three).  Per element the expressions are the persistent
+// kernel's; rho is summed from s and t instead of from the rounded r.
 template <int K, int V, typename TI>
-__device__ void vphase_bi_p(const Inst& I, int row_lo, int row_hi, const Stage<TI>& g, const TI* r, const TI* pold,
-                            TI* pnew, const TI* vold, TI* vnew, bool first, const Shared& sh,
-                            vec<double, V> (&acc)[1]) {
+__device__ void vphase_bi_p(const Inst& I, int row_lo, int row_hi, const Stage<TI>& g, const TI* r, TI* p, TI* v,
+                            bool first, vec<double, V> (&acc)[1]) {
   using TV = vec<TI, V>;
   using L = VLay<K, V>;
-  constexpr int U = vunroll<V>();
+  constexpr int U = vunroll_bi<V>();
   const int j0 = ((int)threadIdx.x % L::G) * V;
   TI* rh = (TI*)I.rh;  // own rows only
   TI* xi = (TI*)I.xi;
   const TV zt = splat<V>((TI)0);
-  const TV be = first ? zt : colv<TI, V>(sh.beta, j0);
-  const TV om = first ? zt : colv<TI, V>(sh.omega, j0);
+  const TI* src = first ? r : p;  // p is written on own rows only in the first iteration
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
-    const TV ro = vld<V>(r + o);
+    TV ho;
     if (first) {
+      ho = vld<V>(r + o);
       vst<V>(I.x + o, vld<V>(I.x + o) + to_d<V>(vld<V>(xi + o)));
       vst<V>(xi + o, zt);
-      vst<V>(rh + o, ro);
+      vst<V>(rh + o, ho);
+      vst<V>(p + o, ho);
+    } else {
+      ho = vld<V>(rh + o);
     }
     const int lr = row - g.row_lo;
-    const TV va = vgather<U>(g.rp[lr], g.rp[lr + 1], zt, [&](int s) {
-      const size_t oc = (size_t)g.ci[s] * K + j0;
-      const TV rcv = vld<V>(r + oc);
-      const TV pc = first ? rcv : (rcv + be * (vld<V>(pold + oc) - om * vld<V>(vold + oc)));
-      return g.w[s] * pc;
-    });
-    const TV pn = first ? ro : (ro + be * (vld<V>(pold + o) - om * vld<V>(vold + o)));
-    vst<V>(pnew + o, pn);
-    vst<V>(vnew + o, va);
-    acc[0] += to_d<V>(first ? ro : vld<V>(rh + o)) * to_d<V>(va);
+    const TV va = vgather<U>(g.rp[lr], g.rp[lr + 1], zt,
+                             [&](int s) { return g.w[s] * vld<V>(src + (size_t)g.ci[s] * K + j0); });
+    vst<V>(v + o, va);
+    acc[0] += to_d<V>(ho) * to_d<V>(va);
   }
 }
 
-// BiCGSTAB phase S: s = r - alpha v; t = S D^-1 s; <t,s>, <t,t>, <s,s>.
+// BiCGSTAB phase S: s = r - alpha v; t = S D^-1 s; <t,s>, <t,t>, <s,s>,
+// <rh,s>, <rh,t>.
 template <int K, int V, typename TI>
-__device__ void vphase_bi_s(const Inst& I, int row_lo, int row_hi, const Stage<TI>& g, const TI* r, const TI* vnew,
-                            const Shared& sh, vec<double, V> (&acc)[3]) {
+__device__ void vphase_bi_s(const Inst& I, int row_lo, int row_hi, const Stage<TI>& g, const TI* r, const TI* v,
+                            const Shared& sh, vec<double, V> (&acc)[5]) {
   using TV = vec<TI, V>;
+  using DV = vec<double, V>;
   using L = VLay<K, V>;
-  constexpr int U = vunroll<V>();
+  constexpr int U = vunroll_bi<V>();
   const int j0 = ((int)threadIdx.x % L::G) * V;
   TI* sv = (TI*)I.s;  // own rows only
   TI* tv = (TI*)I.t;  // own rows only
+  const TI* rh = (const TI*)I.rh;
   const TV zt = splat<V>((TI)0);
   const TV al = colv<TI, V>(sh.alpha, j0);
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
-    const TV so = vld<V>(r + o) - al * vld<V>(vnew + o);
+    const TV so = vld<V>(r + o) - al * vld<V>(v + o);
+    const TV ho = vld<V>(rh + o);
     const int lr = row - g.row_lo;
     const TV ta = vgather<U>(g.rp[lr], g.rp[lr + 1], zt, [&](int s) {
       const size_t oc = (size_t)g.ci[s] * K + j0;
-      return g.w[s] * (vld<V>(r + oc) - al * vld<V>(vnew + oc));
+      return g.w[s] * (vld<V>(r + oc) - al * vld<V>(v + oc));
     });
     vst<V>(sv + o, so);
     vst<V>(tv + o, ta);
-    acc[0] += to_d<V>(ta) * to_d<V>(so);
-    acc[1] += to_d<V>(ta) * to_d<V>(ta);
-    acc[2] += to_d<V>(so) * to_d<V>(so);
+    const DV sd = to_d<V>(so), td = to_d<V>(ta), hd = to_d<V>(ho);
+    acc[0] += td * sd;
+    acc[1] += td * td;
+    acc[2] += sd * sd;
+    acc[3] += hd * sd;
+    acc[4] += hd * td;
   }
 }
 
-// BiCGSTAB phase X: xi += D^-1 (alpha p + omega s); r = s - omega t; <rh,r>, <r,r>.
+// BiCGSTAB phase X: xi += D^-1 (alpha p + omega s); r = s - omega t;
+// p = r + beta (p - omega v); <r,r>.
 template <int K, int V, typename TI>
-__device__ void vphase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, const TI* p, const Shared& sh,
-                            vec<double, V> (&acc)[2]) {
+__device__ void vphase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, TI* p, const TI* v, const Shared& sh,
+                            vec<double, V> (&acc)[1]) {
   using TV = vec<TI, V>;
   using L = VLay<K, V>;
   const int j0 = ((int)threadIdx.x % L::G) * V;
   TI* xi = (TI*)I.xi;
-  const TI* rh = (const TI*)I.rh;
   const TI* sv = (const TI*)I.s;
   const TI* tv = (const TI*)I.t;
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const TV al = colv<TI, V>(sh.alpha, j0);
   const TV om = colv<TI, V>(sh.omega, j0);
+  const TV be = colv<TI, V>(sh.beta, j0);
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
     const TI d = dinv[row];
     const TV so = vld<V>(sv + o);
-    vst<V>(xi + o, vld<V>(xi + o) + d * (al * vld<V>(p + o) + om * so));
-    const TV rr = so - om * vld<V>(tv + o);
+    const TV to = vld<V>(tv + o);
+    const TV po = vld<V>(p + o);
+    const TV vo = vld<V>(v + o);
+    vst<V>(xi + o, vld<V>(xi + o) + d * (al * po + om * so));
+    const TV rr = so - om * to;
     vst<V>(r + o, rr);
-    acc[0] += to_d<V>(vld<V>(rh + o)) * to_d<V>(rr);
-    acc[1] += to_d<V>(rr) * to_d<V>(rr);
+    vst<V>(p + o, rr + be * (po - om * vo));
+    acc[0] += to_d<V>(rr) * to_d<V>(rr);
   }
 }
 
-// Per-lane vector partial sums -> the block's sums at sh.red[q K + j], summed
-// over the lanes of each column in a fixed order.
+// Per-lane vector partial sums acc[0 .. NS) -> the block's sums at
+// sh.red[q K + j], summed over the lanes of each column in a fixed order.
 template <int K, int V, int NS>
-__device__ inline void vblock_partials(const vec<double, V> (&acc)[NS], Shared& sh) {
+__device__ inline void vblock_partials(const vec<double, V>* acc, Shared& sh) {
+  static_assert(NS <= kMaxSlots, "block sums per round");
   constexpr int G = K / V;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
