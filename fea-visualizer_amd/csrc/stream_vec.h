@@ -450,11 +450,22 @@ __device__ void vphase_cg_b(const Inst& I, int row_lo, int row_hi, TI* r, const 
 // forms the next search direction p = r + beta (p - omega v) on its own rows:
 //   P: v = S D^-1 p; <rh, v>          (first iteration: p = rh = r, x += xi)
 //   S: s = r - alpha v; t = S D^-1 s; <t,s>, <t,t>, <s,s>, <rh,s>, <rh,t>
-//   X: xi += D^-1 (alpha p + omega s); r = s - omega t;
-//      p = r + beta (p - omega v); <r,r>
+//   X: s = r - alpha v again; xi += D^-1 (alpha p + omega s);
+//      r = s - omega t; p = r + beta (p - omega v); <r,r>
 // Phase P gathers one vector and S two (the persistent kernel's phase P
-// rebuilds p from three).  Per element the expressions are the persistent
-// kernel's; rho is summed from s and t instead of from the rounded r.
+// rebuilds p from three), and s is never stored: phase X reads v anyway.
+// Per element the expressions are the persistent kernel's (s rounded once
+// more, below); rho is summed from s and t instead of from the rounded r.
+
+// s = r - alpha v, product and difference rounded separately (no fused
+// multiply-add), so that phases S and X form the same values.
+template <typename TI, int V>
+__device__ inline vec<TI, V> bi_s(const vec<TI, V>& r, const vec<TI, V>& al, const vec<TI, V>& v) {
+#pragma clang fp contract(off)
+  const vec<TI, V> av = al * v;
+  return r - av;
+}
+
 template <int K, int V, typename TI>
 __device__ void vphase_bi_p(const Inst& I, int row_lo, int row_hi, const Stage<TI>& g, const TI* r, TI* p, TI* v,
                             bool first, vec<double, V> (&acc)[1]) {
@@ -496,21 +507,19 @@ __device__ void vphase_bi_s(const Inst& I, int row_lo, int row_hi, const Stage<T
   using L = VLay<K, V>;
   constexpr int U = vunroll_bi<V>();
   const int j0 = ((int)threadIdx.x % L::G) * V;
-  TI* sv = (TI*)I.s;  // own rows only
   TI* tv = (TI*)I.t;  // own rows only
   const TI* rh = (const TI*)I.rh;
   const TV zt = splat<V>((TI)0);
   const TV al = colv<TI, V>(sh.alpha, j0);
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
-    const TV so = vld<V>(r + o) - al * vld<V>(v + o);
+    const TV so = bi_s<TI, V>(vld<V>(r + o), al, vld<V>(v + o));
     const TV ho = vld<V>(rh + o);
     const int lr = row - g.row_lo;
     const TV ta = vgather<U>(g.rp[lr], g.rp[lr + 1], zt, [&](int s) {
       const size_t oc = (size_t)g.ci[s] * K + j0;
-      return g.w[s] * (vld<V>(r + oc) - al * vld<V>(v + oc));
+      return g.w[s] * bi_s<TI, V>(vld<V>(r + oc), al, vld<V>(v + oc));
     });
-    vst<V>(sv + o, so);
     vst<V>(tv + o, ta);
     const DV sd = to_d<V>(so), td = to_d<V>(ta), hd = to_d<V>(ho);
     acc[0] += td * sd;
@@ -521,8 +530,8 @@ __device__ void vphase_bi_s(const Inst& I, int row_lo, int row_hi, const Stage<T
   }
 }
 
-// BiCGSTAB phase X: xi += D^-1 (alpha p + omega s); r = s - omega t;
-// p = r + beta (p - omega v); <r,r>.
+// BiCGSTAB phase X: s = r - alpha v; xi += D^-1 (alpha p + omega s);
+// r = s - omega t; p = r + beta (p - omega v); <r,r>.
 template <int K, int V, typename TI>
 __device__ void vphase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, TI* p, const TI* v, const Shared& sh,
                             vec<double, V> (&acc)[1]) {
@@ -530,7 +539,6 @@ __device__ void vphase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, TI* p,
   using L = VLay<K, V>;
   const int j0 = ((int)threadIdx.x % L::G) * V;
   TI* xi = (TI*)I.xi;
-  const TI* sv = (const TI*)I.s;
   const TI* tv = (const TI*)I.t;
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const TV al = colv<TI, V>(sh.alpha, j0);
@@ -539,10 +547,10 @@ __device__ void vphase_bi_x(const Inst& I, int row_lo, int row_hi, TI* r, TI* p,
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
     const TI d = dinv[row];
-    const TV so = vld<V>(sv + o);
+    const TV vo = vld<V>(v + o);
+    const TV so = bi_s<TI, V>(vld<V>(r + o), al, vo);  // phase S's s, bit for bit
     const TV to = vld<V>(tv + o);
     const TV po = vld<V>(p + o);
-    const TV vo = vld<V>(v + o);
     vst<V>(xi + o, vld<V>(xi + o) + d * (al * po + om * so));
     const TV rr = so - om * to;
     vst<V>(r + o, rr);
