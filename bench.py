@@ -80,8 +80,8 @@ def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False, fused=
     matrix twice and 17 vectors on the persistent kernel (p: r, p_old, v_old,
     rh read, p, v written; s: r, v read, s, t written; x: xi, p, s, t, rh
     read, xi, r written), 15 on the streaming engine's multi-column layout
-    (stream: p read, v written; s: r, v, rh read, s, t written; x: xi, p, s,
-    t, v read, xi, r, p written).
+    (stream: p: p, rh read, v written; s: r, v, rh read, t written; x: xi,
+    r, t, p, v read, xi, r, p written).
     fused: first-order equations on the persistent kernel, whose residual
     pass also reads M and the history combination and writes M x and S x0,
     so the update + right-hand side phase reads no matrix."""
