@@ -1309,7 +1309,18 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   const int max_sweeps = b->opts.max_sweeps > 0 ? b->opts.max_sweeps : 30;
   const int max_inner = b->opts.max_inner > 0 ? b->opts.max_inner : 200;
   FEM_HIP(hipEventRecord(b->ev0, b->stream));
-  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream);
+  int turned = 0;
+  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream, &turned);
+  // the device turned the predictor history of first-order instances once per
+  // step (k_step_end): turn the host's pointers as often
+  for (auto& h : b->insts) {
+    Inst& I = h.dev;
+    if (!first_order_eq(I.eq) || I.extrap < 2) continue;
+    for (int t = 0; t < turned % I.extrap; t++) {
+      turn_history(&h.d_a, &h.d_v, &h.d_h3, &h.d_h4, I.extrap);
+      turn_history(&I.a, &I.v, &I.h3, &I.h4, I.extrap);
+    }
+  }
   if (rc) return rc;
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
   rc = read_info(b);

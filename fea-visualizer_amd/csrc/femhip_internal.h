@@ -29,6 +29,20 @@ enum Eq : int {
   EQ_SOLVE = 5,  // plain solve S x = b (b supplied), used by the adjoint
 };
 
+// Equations whose new state is the solve's solution; their predictor
+// history (Inst::extrap levels) lives in a, v, h3, h4, newest first.
+__host__ __device__ inline bool first_order_eq(int eq) { return eq == EQ_HEAT || eq == EQ_ADVDIFF || eq == EQ_GS; }
+
+// One turn of that history by pointer: the oldest of the e slots becomes the
+// newest (the streaming engine's update has written u_n into it).
+__host__ __device__ inline void turn_history(double** a, double** v, double** h3, double** h4, int e) {
+  double** s[4] = {a, v, h3, h4};
+  if (e < 2 || e > 4) return;
+  double* t = *s[e - 1];
+  for (int i = e - 1; i > 0; i--) *s[i] = *s[i - 1];
+  *s[0] = t;
+}
+
 // Status bits accumulated per instance.
 enum : int {
   ST_NOT_CONVERGED = 1,

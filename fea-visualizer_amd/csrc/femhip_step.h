@@ -38,7 +38,8 @@ struct InstHost;
 void stream_plan_free(StreamPlan* p);
 int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
                       hipStream_t st, StreamPlan** out);
-int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps, hipStream_t st);
+int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps, hipStream_t st,
+               int* turned);
 // Column counts whose stream phases use the multi-column layout with the
 // staged pattern (one value column, Inst::SDi present).
 bool stream_vec_layout(int k);

@@ -134,6 +134,45 @@ __device__ void run_scalar(const Inst& I, const SArgs& a, Shared& sh, int b, int
   }
 }
 
+// State update of heat, advection-diffusion and Gray-Scott (phase_update's
+// arithmetic), with the predictor history turned by pointer instead of by
+// copies: a row moves u_n into the oldest history slot, which k_step_end then
+// makes the newest (a), and stores the new solution into u.  x + xi is not
+// stored back, since the next right-hand side phase rewrites x and xi.
+// guess: also the next step's initial guess of order lvn into x, which the
+// next right-hand side phase gathers (vphase_rhs).
+template <int K, typename TI>
+__device__ void stream_update_first(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags, int step,
+                                    int lvn, bool guess) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  const TI GA* xi = gp((const TI*)I.xi);
+  const int e = I.extrap;
+  double GA* oldest = gp(e == 1 ? I.a : e == 2 ? I.v : e == 3 ? I.h3 : I.h4);  // e >= 1
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0;
+  if (guess) pred_weights(lvn, w0, w1, w2, w3, w4);
+  int f = 0;
+  double* traj = nullptr;  // trajectory record of this step, if any
+  if (I.traj && (step + 1) % I.traj_every == 0)
+    traj = I.traj + (size_t)((step + 1) / I.traj_every - 1) * I.n * K;
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    double xf = gp(I.x)[o] + (double)xi[o];
+    const double uo = gp(I.u)[o];
+    const double ao = e >= 2 ? gp(I.a)[o] : 0.0;
+    const double vo = e >= 3 ? gp(I.v)[o] : 0.0;
+    const double ho = e >= 4 ? gp(I.h3)[o] : 0.0;
+    if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
+    if (guess) gp(I.x)[o] = w0 * xf + w1 * uo + w2 * ao + w3 * vo + w4 * ho;
+    if (e >= 1) oldest[o] = uo;
+    gp(I.u)[o] = xf;
+    if (traj) gp(traj)[o] = xf;
+    if (xf > 1e4) f |= ST_UNSTABLE;
+    if (!isfinite(xf)) f |= ST_NONFINITE;
+  }
+  if (f) atomicOr(flags, f);
+}
+
 // V adjacent columns per lane (stream_vec.h); g: the staged pattern of the
 // gathering Krylov phases.
 template <int K, int V, typename TI, int PH>
@@ -217,7 +256,11 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   if constexpr (PH == PH_UPD) {
     __syncthreads();
     const int lvn = (a.more && I.warm) ? min(I.extrap, I.hist + a.step + 1) : -1;
-    phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
+    if (first_order_eq(I.eq))  // the guess goes to x where vphase_rhs gathers it
+      stream_update_first<K, TI>(I, row_lo, row_hi, sh, I.status, a.step, lvn,
+                                 vec_layout<K>() && lvn > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF));
+    else
+      phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
   } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
     // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
     extern __shared__ int dyn[];
@@ -458,12 +501,15 @@ __global__ void k_sweep_begin(SState* ss, const Inst* insts, int ninst, int k, i
   S.sweep_done = 0;
 }
 
-// End of a step: counters, convergence status, last residual.
-__global__ void k_step_end(SState* ss, const Inst* insts, int ninst, int k) {
+// End of a step: counters, convergence status, last residual, and the turn
+// of the predictor history (stream_update_first; the host turns its copies
+// of the pointers by the same count after the run).
+__global__ void k_step_end(SState* ss, Inst* insts, int ninst, int k) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   SState& S = ss[i];
-  const Inst& I = insts[i];
+  Inst& I = insts[i];
+  if (first_order_eq(I.eq)) turn_history(&I.a, &I.v, &I.h3, &I.h4, I.extrap);
   int notconv = 0;
   for (int j = 0; j < k; j++) {
     notconv |= S.active[j];
@@ -793,9 +839,11 @@ int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, bool read)
 }  // namespace
 
 // Advance every instance by nsteps with the streaming engine.  The instance
-// descriptors are already on the device (d_insts).
+// descriptors are already on the device (d_insts); *turned counts the steps
+// whose end (and turn of the predictor history) was queued.
 int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps,
-               hipStream_t st) {
+               hipStream_t st, int* turned) {
+  *turned = 0;
   SArgs a{};
   a.insts = d_insts;
   a.ss = p->d_ss;
@@ -892,9 +940,10 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     }
     a.more = step + 1 < nsteps;
     STREAM_PH(PH_UPD, full);
-    hipLaunchKernelGGL(k_step_end, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts, p->ninst,
-                       p->k);
+    hipLaunchKernelGGL(k_step_end, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss,
+                       const_cast<Inst*>(d_insts), p->ninst, p->k);
     FEM_HIP(hipGetLastError());
+    ++*turned;
   }
 #undef STREAM_PH
   hipLaunchKernelGGL(k_hist_store, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, d_insts, p->ninst, nsteps);

@@ -158,8 +158,8 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
   const int lv = warm ? min(I.extrap, I.hist + step) : 0;  // predictor order this step
   const bool ex = lv > 0;
   // after the first step of a launch the update wrote this step's guess into
-  // xnext (one-column-per-lane update phase, stream.hip)
-  const bool pre = ex && step > 0 && I.xnext && I.eq != EQ_GS;
+  // x (stream_update_first, stream.hip)
+  const bool pre = ex && step > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -174,14 +174,13 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
       case EQ_HEAT:
       case EQ_ADVDIFF:
       case EQ_GS: {  // b = (M/dt) u                                  CPUSolver.cpp:78,96
-        if (pre) {  // the previous update formed the guess (xnext): gather it alone
+        if (pre) {  // the previous update formed the guess in x: gather it alone
           const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
             const size_t oc = (size_t)ci[s] * K + j0;
-            return P2<DV>{M64[s] * vld<V>(I.u + oc), S64[s] * vld<V>(I.xnext + oc)};
+            return P2<DV>{M64[s] * vld<V>(I.u + oc), S64[s] * vld<V>(I.x + oc)};
           });
           b = g.a * inv_dt;
           sx = g.b;
-          x0 = vld<V>(I.xnext + o);
           break;
         }
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
@@ -252,7 +251,7 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
     }
     const DV r64 = b - sx;
     vst<V>(I.b + o, b);
-    vst<V>(I.x + o, x0);
+    if (!pre) vst<V>(I.x + o, x0);  // pre: x holds the guess already (and is gathered)
     const TV rr = to_i<TI, V>(r64);
     vst<V>(r + o, rr);
     vst<V>(xi + o, splat<V>((TI)0));
