@@ -676,6 +676,61 @@ def test_predictors(gpu, case, pred):
         assert rel(sol.get_state_unknowns(), u) < RTOL, n
 
 
+def test_history_across_engines_and_autotune(gpu):
+    """The streaming engine turns the predictor history by pointer, the
+    persistent kernel shifts it by copies: a cubic-predictor trajectory that
+    starts on the persistent kernel, goes through the autotuner's probes of
+    both engines (each restoring the state) and continues on the chosen one
+    stays on the oracle's trajectory, and so do stream launches of three, one
+    and two steps (turn counts that the three history slots do not divide)."""
+    print("case history across engines", flush=True)
+    k, vel, prm = 16, (0.5, 0.3, 0.0), [1e-3, 3e-2]
+    rng = np.random.default_rng(21)
+    ctxs, systems, states = [], [], []
+    for side, seed in ((17, 3), (11, 4)):
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+        systems.append(oracle.System(m, bc=0, velocity=vel, normalize=False))
+        states.append(rng.normal(size=(systems[-1].n, k)))
+
+    def oracle_steps(n):
+        for i, s in enumerate(systems):
+            for j in range(k):
+                _, states[i][:, j], _, _ = s.run(oracle.ADVDIFF, prm, n, states[i][:, j].copy(),
+                                                  krylov=oracle.BICGSTAB, tol=1e-14)
+
+    def check(b, tag):
+        for i, s in enumerate(systems):
+            assert rel(b.get_state_unknowns(i).reshape(s.n, k), states[i]) < RTOL, (tag, i)
+
+    def batch(engine):
+        b = Batch(ctxs, SolverOptions(nrhs=k, engine=engine, predictor=_lib.FEM_PREDICTOR_CUBIC))
+        for i in range(2):
+            b.set_equation(i, AdvectionDiffusionParameters(*prm))
+        return b
+
+    b = batch(_lib.FEM_ENGINE_PERSISTENT)
+    for i in range(2):
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+    assert b.advance(3) == _lib.FEM_OK
+    oracle_steps(3)
+    check(b, "persistent")
+    b.autotune(2)
+    assert b.advance(4) == _lib.FEM_OK
+    oracle_steps(4)
+    check(b, "after autotune")
+    b.close()
+    # stream launches of 3 and 1 steps (history of 3 slots turned 3 and 4 times)
+    b = batch(_lib.FEM_ENGINE_STREAM)
+    for i in range(2):
+        b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+    for n in (3, 1, 2):
+        assert b.advance(n) == _lib.FEM_OK
+        oracle_steps(n)
+        check(b, f"stream {n}")
+    b.close()
+
+
 def test_stream_engine_adjoint_matches_persistent(gpu):
     from femhip import Adjoint
     m = meshes.jittered_square(24, seed=12)
