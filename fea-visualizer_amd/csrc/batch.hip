@@ -1187,21 +1187,27 @@ int batch_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
 // barrier-slot reset kernel (a fill kernel, no memset node) and the stepping
 // kernel.  Replays reuse the plan's device arrays; a plan rebuilt since
 // (new equation, new matrices, autotuning) makes the graph stale.
+static int stream_graph_capture(fem_batch* b, int nsteps);
+static int stream_graph_launch(fem_batch* b, fem_step_stats* stats);
+
 int batch_graph_capture(fem_batch* b, int nsteps) {
   if (nsteps <= 0) {
     set_error("graph capture: nsteps must be positive");
     return FEM_ERR_INVALID;
   }
-  if (b->opts.engine != FEM_ENGINE_PERSISTENT) {
-    set_error("graph capture records the persistent engine's launch; the other engines are host-driven");
+  if (b->opts.engine != FEM_ENGINE_PERSISTENT && b->opts.engine != FEM_ENGINE_STREAM) {
+    set_error("graph capture records the persistent or the streaming engine; the plain and GL-compatible engines "
+              "are host-driven");
     return FEM_ERR_INVALID;
   }
-  int rc = batch_prepare(b, true);
-  if (rc) return rc;
   if (b->graph) {
     FEM_HIP(hipGraphExecDestroy(b->graph));
     b->graph = nullptr;
   }
+  b->graph_splan = nullptr;
+  if (b->opts.engine == FEM_ENGINE_STREAM) return stream_graph_capture(b, nsteps);
+  int rc = batch_prepare(b, true);
+  if (rc) return rc;
   hipGraph_t g = nullptr;
   FEM_HIP(hipGraphCreate(&g, 0));
   int ninst = (int)b->insts.size();
@@ -1243,6 +1249,13 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
     set_error("graph launch: no graph was captured");
     return FEM_ERR_STATE;
   }
+  if (b->graph_splan) {
+    if (b->opts.engine != FEM_ENGINE_STREAM) {
+      set_error("graph launch: the batch no longer runs the streaming engine; capture again");
+      return FEM_ERR_STATE;
+    }
+    return stream_graph_launch(b, stats);
+  }
   if (b->opts.engine != FEM_ENGINE_PERSISTENT) {
     set_error("graph launch: the batch no longer runs the persistent engine; capture again");
     return FEM_ERR_STATE;
@@ -1271,8 +1284,9 @@ int batch_graph_launch(fem_batch* b, fem_step_stats* stats) {
 
 // Streaming engine: every phase a launch over the row blocks of all
 // instances (stream.hip).  Instances must share the column count, the inner
-// precision and the Krylov method, as in the persistent plan.
-int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
+// precision and the Krylov method, as in the persistent plan.  Set-up of a
+// run: checks, the descriptors uploaded, the row-block plan (re)built.
+static int stream_setup(fem_batch* b, int* max_sweeps, int* max_inner) {
   int rc = batch_prepare(b, false);
   if (rc) return rc;
   const int ninst = (int)b->insts.size();
@@ -1305,14 +1319,18 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
     rc = stream_plan_build(b->insts, b->d_sinsts, k, fp32, kr, rows, b->stream, &b->splan);
     if (rc) return rc;
     b->splan_key = (long long)rows * 1000003LL + k * 131 + kr * 7 + fp32;
+    b->splan_gen++;
+    b->sobs_sweeps = 0;  // a new plan: a graph's budget is sized from its own runs
+    for (int& v : b->sobs_iters) v = 0;
   }
-  const int max_sweeps = b->opts.max_sweeps > 0 ? b->opts.max_sweeps : 30;
-  const int max_inner = b->opts.max_inner > 0 ? b->opts.max_inner : 200;
-  FEM_HIP(hipEventRecord(b->ev0, b->stream));
-  int turned = 0;
-  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream, &turned);
-  // the device turned the predictor history of first-order instances once per
-  // step (k_step_end): turn the host's pointers as often
+  *max_sweeps = b->opts.max_sweeps > 0 ? b->opts.max_sweeps : 30;
+  *max_inner = b->opts.max_inner > 0 ? b->opts.max_inner : 200;
+  return 0;
+}
+
+// The device turned the predictor history of first-order instances once per
+// step (k_step_end): turn the host's pointers as often.
+static void stream_turn_host(fem_batch* b, int turned) {
   for (auto& h : b->insts) {
     Inst& I = h.dev;
     if (!first_order_eq(I.eq) || I.extrap < 2) continue;
@@ -1321,8 +1339,93 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
       turn_history(&I.a, &I.v, &I.h3, &I.h4, I.extrap);
     }
   }
+}
+
+int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
+  int max_sweeps = 0, max_inner = 0;
+  int rc = stream_setup(b, &max_sweeps, &max_inner);
   if (rc) return rc;
+  FEM_HIP(hipEventRecord(b->ev0, b->stream));
+  int turned = 0;
+  StreamCtl ctl{};
+  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream, &turned, &ctl);
+  stream_turn_host(b, turned);
+  if (rc) return rc;
+  // sweeps and iterations this plan needed, for a graph's budget
+  b->sobs_sweeps = std::max(b->sobs_sweeps, ctl.obs_sweeps);
+  for (int q = 0; q < kBudgetSweeps; q++) b->sobs_iters[q] = std::max(b->sobs_iters[q], ctl.obs_iters[q]);
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
+  rc = read_info(b);
+  if (rc) return rc;
+  float ms = 0.f;
+  FEM_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  return parse_stats(b, stats, ms);
+}
+
+// Record one streaming advance of nsteps as a graph: every step issues the
+// sweeps and iterations the eager runs of this plan needed, plus a margin
+// (a quarter more iterations per sweep, at least 2, and one more sweep),
+// over a grid of 8 workgroups per CU; nothing is read on the host.  An
+// instance that would need more stops there and reports FEM_NOT_CONVERGED.
+static int stream_graph_capture(fem_batch* b, int nsteps) {
+  int max_sweeps = 0, max_inner = 0;
+  int rc = stream_setup(b, &max_sweeps, &max_inner);
+  if (rc) return rc;
+  if (b->sobs_sweeps == 0) {
+    set_error("graph capture (streaming engine): advance the batch once first, so that the recorded sweeps and "
+              "iterations can be sized from what its steps need");
+    return FEM_ERR_STATE;
+  }
+  int dev = 0, cus = 0;
+  FEM_HIP(hipGetDevice(&dev));
+  FEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  StreamCtl ctl{};
+  ctl.budget = true;
+  ctl.grid = 8 * std::max(cus, 1);
+  ctl.sweeps = std::min({kBudgetSweeps, max_sweeps, b->sobs_sweeps + 1});
+  for (int sw = 0; sw < ctl.sweeps; sw++) {
+    const int o = std::max(1, b->sobs_iters[std::min(sw, b->sobs_sweeps - 1)]);
+    ctl.iters[sw] = std::min(max_inner, o + std::max(2, (o + 3) / 4));
+  }
+  FEM_HIP(hipStreamSynchronize(b->stream));  // the upload of stream_setup is done before the capture
+  FEM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+  int turned = 0;
+  rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream, &turned, &ctl);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (!rc && ee != hipSuccess) rc = hip_fail(ee, "hipStreamEndCapture");
+  if (!rc) {
+    const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      b->graph = nullptr;
+      rc = hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  if (g) (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  b->graph_nsteps = nsteps;
+  b->graph_splan = b->splan;
+  b->graph_sgen = b->splan_gen;
+  b->graph_sinsts = b->d_sinsts;
+  b->graph_budget_sweeps = ctl.sweeps;
+  return 0;
+}
+
+// Replay of a streaming graph: the descriptors (with the host's turn of the
+// history) are uploaded first; the graph turns the device's copy nsteps
+// times, and the host's pointers follow.
+static int stream_graph_launch(fem_batch* b, fem_step_stats* stats) {
+  int max_sweeps = 0, max_inner = 0;
+  int rc = stream_setup(b, &max_sweeps, &max_inner);
+  if (rc) return rc;
+  if (b->splan != b->graph_splan || b->splan_gen != b->graph_sgen || b->d_sinsts != b->graph_sinsts) {
+    set_error("graph launch: the streaming plan changed since the capture; capture again");
+    return FEM_ERR_STATE;
+  }
+  FEM_HIP(hipEventRecord(b->ev0, b->stream));
+  FEM_HIP(hipGraphLaunch(b->graph, b->stream));
+  FEM_HIP(hipEventRecord(b->ev1, b->stream));
+  stream_turn_host(b, b->graph_nsteps);
   rc = read_info(b);
   if (rc) return rc;
   float ms = 0.f;

@@ -165,15 +165,6 @@ inline T* dalloc(size_t count, int* rc) {
 
 inline unsigned int div_up(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }
 
-// Launched block -> position in a list of n blocks, so that each XCD
-// (workgroups are dealt to the 8 XCDs round-robin) walks one contiguous
-// range: neighbouring rows, whose data the blocks share, then meet in the
-// same XCD's L2.  A bijection on [0, n).
-__device__ inline int xcd_chunk(int b, int n) {
-  const int x = b & 7, q = n >> 3, r = n & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
-
 // Device exclusive scan (int32 -> int32 offsets, total returned), scan.hip.
 int exclusive_scan_i32(const int* in, int* out, int n, int* total, hipStream_t st);
 

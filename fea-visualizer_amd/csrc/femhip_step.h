@@ -38,8 +38,22 @@ struct InstHost;
 void stream_plan_free(StreamPlan* p);
 int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
                       hipStream_t st, StreamPlan** out);
+// Control of a streaming run.  Eager runs (budget == false) stop each sweep
+// from host reads of the device's counts and record the most sweeps a step
+// took and the most iterations each sweep took (obs_*).  A budgeted run, the
+// one a graph records, reads nothing on the host: every step issues
+// `sweeps` sweeps of iters[sw] iterations over a grid of `grid` workgroups,
+// and the launches past an instance's end exit at once on the device.
+constexpr int kBudgetSweeps = 8;
+struct StreamCtl {
+  bool budget = false;
+  int sweeps = 0, grid = 0;
+  int iters[kBudgetSweeps] = {};
+  int obs_sweeps = 0;
+  int obs_iters[kBudgetSweeps] = {};
+};
 int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps, hipStream_t st,
-               int* turned);
+               int* turned, StreamCtl* ctl);
 // Column counts whose stream phases use the multi-column layout with the
 // staged pattern (one value column, Inst::SDi present).
 bool stream_vec_layout(int k);
@@ -133,12 +147,24 @@ struct fem_batch {
   int sinsts_n = 0;
   int stream_rows = 0;  // rows per block (0 = by column count)
   long long splan_key = -1;
+  long long splan_gen = 0;  // counts plan rebuilds
   // recorded advance (persistent engine): valid while the plan it was built
   // on is current
   long long plan_gen = 0;
   hipGraphExec_t graph = nullptr;
   int graph_nsteps = 0;
   long long graph_gen = -1;
+  // a streaming graph: the plan (and its generation) and descriptor array it
+  // was recorded on (graph_splan == nullptr: a persistent graph), and its
+  // sweeps per step
+  femhip::StreamPlan* graph_splan = nullptr;
+  long long graph_sgen = -1;
+  femhip::Inst* graph_sinsts = nullptr;
+  int graph_budget_sweeps = 0;
+  // most sweeps a step and most iterations each sweep took in eager
+  // streaming runs of the current plan (a streaming graph's budget)
+  int sobs_sweeps = 0;
+  int sobs_iters[femhip::kBudgetSweeps] = {};
 };
 
 struct fem_solver {

@@ -72,6 +72,7 @@ struct SArgs {
   const int* act_inst;  // reduction block -> instance (null: every instance)
   const int* act_cnt;   // {instances, blocks} of the lists (device); the grid may be larger
   const int* blk_end;   // [nblocks] end row
+  int nall;             // row blocks of the plan (launches that cover all of them)
   int stage_rows;       // LDS stage capacity: rows and nonzeros per row block
   int stage_nnz;
 };
@@ -222,69 +223,82 @@ __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int
   }
 }
 
-template <int K, typename TI, int PH>
+// One launch covers nb entries of the block list (every row block when the
+// launch covers all of them).  Workgroup blockIdx sits on XCD x = blockIdx
+// mod 8, which owns one contiguous chunk of the list (halo rows then hit its
+// L2); the XCD's workgroups take the chunk's entries in turn.  With one
+// workgroup per entry (the host knows the list's size) each takes one entry;
+// WALK: a fixed smaller grid (graph replays, where the size is known only on
+// the device) walks several (a separate variant: the loop costs registers).
+template <int K, typename TI, int PH, bool WALK = false>
 __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
   __shared__ Shared sh;
-  int nb = gridDim.x;
-  if (a.act_cnt) {  // the grid is the host's last count, a superset of the current list
-    nb = a.act_cnt[1];
-    if ((int)blockIdx.x >= nb) return;
-  }
-  const int c = xcd_chunk(blockIdx.x, nb);
-  const int b = a.act_blk ? a.act_blk[c] : c;
-  const int i = a.blk_inst[b];
-  const Inst& I = a.insts[i];
-  const SState& S = a.ss[i];
-  // skip rules (uniform per block)
-  if constexpr (PH == PH_RES) {
-    if (a.pass ? !S.redo : S.solve_done) return;
-  } else if constexpr (PH != PH_RHS && PH != PH_UPD) {
-    if (S.sweep_done) return;
-  }
-  if (threadIdx.x < K) {
-    const int j = threadIdx.x;
-    sh.gamma[j] = S.gamma[j];
-    sh.alpha[j] = S.alpha[j];
-    sh.beta[j] = S.beta[j];
-    sh.omega[j] = S.omega[j];
-    sh.bb[j] = S.bb[j];
-    sh.reset[j] = (PH == PH_RES && a.pass) ? S.reset[j] : 0;
-  }
-  const int row_lo = a.blk_row[b];
-  const int row_hi = a.blk_end[b];
-  constexpr int V = vec_cols<K, TI>();
-  if constexpr (PH == PH_UPD) {
-    __syncthreads();
-    const int lvn = (a.more && I.warm) ? min(I.extrap, I.hist + a.step + 1) : -1;
-    if (first_order_eq(I.eq))  // the guess goes to x where vphase_rhs gathers it
-      stream_update_first<K, TI>(I, row_lo, row_hi, sh, I.status, a.step, lvn,
-                                 vec_layout<K>() && lvn > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF));
-    else
-      phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
-  } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
-    // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
-    extern __shared__ int dyn[];
-    const int GA* rp = gp(I.rp);
-    const int s_lo = rp[row_lo], s_hi = rp[row_hi];
-    const int nrow = row_hi - row_lo, nnz = s_hi - s_lo;  // within the stage (plan)
-    int LA* lrp = (int LA*)dyn;
-    int LA* lci = lrp + ((a.stage_rows + 4) & ~3);
-    TI LA* lw = (TI LA*)(lci + ((a.stage_nnz + 3) & ~3));
-    const int GA* ci = gp(I.ci);
-    const TI GA* w = gp((const TI*)(PH == PH_CG_A ? I.Si : I.SDi));
-    for (int t = threadIdx.x; t <= nrow; t += kBlock) lrp[t] = rp[row_lo + t] - s_lo;
-    for (int t = threadIdx.x; t < nnz; t += kBlock) {
-      lci[t] = ci[s_lo + t];
-      lw[t] = w[s_lo + t];
+  const int nb = a.act_cnt ? a.act_cnt[1] : a.nall;
+  const int x = (int)blockIdx.x & 7, q = nb >> 3, r = nb & 7;
+  const int cx = q + (x < r);                      // entries of XCD x
+  const int c0 = x * q + min(x, r);                // its first entry
+  const int gx = ((int)gridDim.x - x + 7) >> 3;    // workgroups on XCD x
+  for (int jx = (int)blockIdx.x >> 3; jx < cx; jx += gx) {
+    const int c = c0 + jx;
+    const int b = a.act_blk ? a.act_blk[c] : c;
+    const int i = a.blk_inst[b];
+    const Inst& I = a.insts[i];
+    const SState& S = a.ss[i];
+    // skip rules (uniform per block)
+    bool skip = false;
+    if constexpr (PH == PH_RES) skip = a.pass ? !S.redo : S.solve_done;
+    else if constexpr (PH != PH_RHS && PH != PH_UPD) skip = S.sweep_done != 0;
+    if (skip) {
+      if constexpr (WALK) continue;
+      else return;
     }
-    __syncthreads();
-    run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
-  } else if constexpr (vec_layout<K>()) {
-    __syncthreads();
-    run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
-  } else {
-    __syncthreads();
-    run_scalar<K, TI, PH>(I, a, sh, b, row_lo, row_hi);
+    if (threadIdx.x < K) {
+      const int j = threadIdx.x;
+      sh.gamma[j] = S.gamma[j];
+      sh.alpha[j] = S.alpha[j];
+      sh.beta[j] = S.beta[j];
+      sh.omega[j] = S.omega[j];
+      sh.bb[j] = S.bb[j];
+      sh.reset[j] = (PH == PH_RES && a.pass) ? S.reset[j] : 0;
+    }
+    const int row_lo = a.blk_row[b];
+    const int row_hi = a.blk_end[b];
+    constexpr int V = vec_cols<K, TI>();
+    if constexpr (PH == PH_UPD) {
+      __syncthreads();
+      const int lvn = (a.more && I.warm) ? min(I.extrap, I.hist + a.step + 1) : -1;
+      if (first_order_eq(I.eq))  // the guess goes to x where vphase_rhs gathers it
+        stream_update_first<K, TI>(I, row_lo, row_hi, sh, I.status, a.step, lvn,
+                                   vec_layout<K>() && lvn > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF));
+      else
+        phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
+    } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
+      // stage the block's pattern: row pointers, columns, S D^-1 (phase A of classic CG: S)
+      extern __shared__ int dyn[];
+      const int GA* rp = gp(I.rp);
+      const int s_lo = rp[row_lo], s_hi = rp[row_hi];
+      const int nrow = row_hi - row_lo, nnz = s_hi - s_lo;  // within the stage (plan)
+      int LA* lrp = (int LA*)dyn;
+      int LA* lci = lrp + ((a.stage_rows + 4) & ~3);
+      TI LA* lw = (TI LA*)(lci + ((a.stage_nnz + 3) & ~3));
+      const int GA* ci = gp(I.ci);
+      const TI GA* w = gp((const TI*)(PH == PH_CG_A ? I.Si : I.SDi));
+      for (int t = threadIdx.x; t <= nrow; t += kBlock) lrp[t] = rp[row_lo + t] - s_lo;
+      for (int t = threadIdx.x; t < nnz; t += kBlock) {
+        lci[t] = ci[s_lo + t];
+        lw[t] = w[s_lo + t];
+      }
+      __syncthreads();
+      run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
+    } else if constexpr (vec_layout<K>()) {
+      __syncthreads();
+      run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
+    } else {
+      __syncthreads();
+      run_scalar<K, TI, PH>(I, a, sh, b, row_lo, row_hi);
+    }
+    if constexpr (!WALK) return;
+    __syncthreads();  // the next entry reuses the scalars and the stage
   }
 }
 
@@ -608,7 +622,20 @@ __global__ void k_expand(const int* blk_inst, const int* inst_blk, const int* ac
 }
 
 template <int K, typename TI>
-const void* phase_fn(int ph) {
+const void* phase_fn(int ph, bool walk) {
+  if (walk) {  // the phases a budgeted run launches over its fixed grid
+    switch (ph) {
+      case PH_RES: return (const void*)k_sphase<K, TI, PH_RES, true>;
+      case PH_PCG_INIT: return (const void*)k_sphase<K, TI, PH_PCG_INIT, true>;
+      case PH_PCG_ITER: return (const void*)k_sphase<K, TI, PH_PCG_ITER, true>;
+      case PH_CG_A: return (const void*)k_sphase<K, TI, PH_CG_A, true>;
+      case PH_CG_B: return (const void*)k_sphase<K, TI, PH_CG_B, true>;
+      case PH_BI_P: return (const void*)k_sphase<K, TI, PH_BI_P, true>;
+      case PH_BI_S: return (const void*)k_sphase<K, TI, PH_BI_S, true>;
+      case PH_BI_X: return (const void*)k_sphase<K, TI, PH_BI_X, true>;
+      default: break;
+    }
+  }
   switch (ph) {
     case PH_RHS: return (const void*)k_sphase<K, TI, PH_RHS>;
     case PH_RES: return (const void*)k_sphase<K, TI, PH_RES>;
@@ -639,12 +666,12 @@ const void* reduce_fn(int ph) {
   }
 }
 
-const void* pick_phase(int k, int fp32, int ph) {
+const void* pick_phase(int k, int fp32, int ph, bool walk) {
   switch (k) {
-    case 1: return fp32 ? phase_fn<1, float>(ph) : phase_fn<1, double>(ph);
-    case 2: return fp32 ? phase_fn<2, float>(ph) : phase_fn<2, double>(ph);
-    case 4: return fp32 ? phase_fn<4, float>(ph) : phase_fn<4, double>(ph);
-    case 16: return fp32 ? phase_fn<16, float>(ph) : phase_fn<16, double>(ph);
+    case 1: return fp32 ? phase_fn<1, float>(ph, walk) : phase_fn<1, double>(ph, walk);
+    case 2: return fp32 ? phase_fn<2, float>(ph, walk) : phase_fn<2, double>(ph, walk);
+    case 4: return fp32 ? phase_fn<4, float>(ph, walk) : phase_fn<4, double>(ph, walk);
+    case 16: return fp32 ? phase_fn<16, float>(ph, walk) : phase_fn<16, double>(ph, walk);
     default: return nullptr;
   }
 }
@@ -779,6 +806,7 @@ namespace {
 struct Cover {
   bool all;
   int nact, nab;
+  bool walk = false;  // a fixed grid walks the lists (budgeted runs)
 };
 
 // Host checks of "anything left" in the Krylov loop: every kCheckEvery-th
@@ -789,7 +817,7 @@ struct Cover {
 constexpr int kCheckEvery = 4;
 
 int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, hipStream_t st) {
-  const void* fn = pick_phase(p->k, p->fp32, ph);
+  const void* fn = pick_phase(p->k, p->fp32, ph, !cv.all && cv.walk);
   if (!fn) {
     set_error("stream engine: unsupported column count");
     return FEM_ERR_INVALID;
@@ -840,16 +868,23 @@ int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, bool read)
 
 // Advance every instance by nsteps with the streaming engine.  The instance
 // descriptors are already on the device (d_insts); *turned counts the steps
-// whose end (and turn of the predictor history) was queued.
+// whose end (and turn of the predictor history) was queued.  ctl: eager or
+// budgeted (graph-recorded) control, and the eager run's observations.
 int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max_inner, int nsteps,
-               hipStream_t st, int* turned) {
+               hipStream_t st, int* turned, StreamCtl* ctl) {
   *turned = 0;
+  const bool budget = ctl->budget;
+  if (budget && (ctl->sweeps < 1 || ctl->sweeps > kBudgetSweeps || ctl->grid < 8)) {
+    set_error("stream engine: bad iteration budget");
+    return FEM_ERR_INVALID;
+  }
   SArgs a{};
   a.insts = d_insts;
   a.ss = p->d_ss;
   a.blk_inst = p->d_blk_inst;
   a.blk_row = p->d_blk_row;
   a.blk_end = p->d_blk_end;
+  a.nall = p->nblocks;
   a.stage_rows = p->stage_rows;
   a.stage_nnz = p->stage_nnz;
   a.partials = p->d_partials;
@@ -859,9 +894,16 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   FEM_HIP(hipGetLastError());
   int rc = 0;
   const Cover full{true, 0, 0};
+  // budgeted: the lists' sizes stay on the device; a fixed grid walks them
+  const Cover fixed{false, p->ninst, std::min(ctl->grid, std::max(p->nblocks, 8)), true};
   Cover cv{};
   const bool multi = p->ninst > 1;  // one instance: its list is all of its blocks or none
   std::vector<int> prev_its(std::max(0, max_sweeps), 0);  // iterations each sweep ran in the last step
+  if (!budget) {
+    ctl->obs_sweeps = 0;
+    for (int& v : ctl->obs_iters) v = 0;
+  }
+  const int nsweeps = budget ? ctl->sweeps : max_sweeps;
 #define STREAM_PH(ph, cover)                                     \
   do {                                                           \
     if ((rc = launch_phase(p, a, (ph), (cover), st))) return rc; \
@@ -872,21 +914,28 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     a.first = 1;
     a.pass = 0;
     STREAM_PH(PH_RHS, full);
-    for (int sw = 0; sw < max_sweeps; sw++) {
+    for (int sw = 0; sw < nsweeps; sw++) {
       hipLaunchKernelGGL(k_sweep_begin, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts,
                          p->ninst, p->k, p->fp32);
       FEM_HIP(hipGetLastError());
-      if ((rc = compact(p, 0, st, &cv, true))) return rc;  // instances that sweep
-      if (cv.nab == 0) break;
+      if ((rc = compact(p, 0, st, &cv, !budget))) return rc;  // instances that sweep
+      if (budget) cv = fixed;
+      else if (cv.nab == 0) break;
       const Cover sweep = cv;  // covers every instance of this sweep
       // one iteration done: refresh the lists (several instances) and, every
       // kCheckEvery-th time, read whether any instance still iterates
+      // (budgeted: refresh the lists only)
       int since = 0, it_run = 0;
       const int expect = sw < (int)prev_its.size() ? prev_its[sw] : 0;
       auto next = [&](bool* stop) -> int {
         ++it_run;
-        const bool check = ++since >= kCheckEvery || (expect > 0 && it_run >= expect - 1);
         *stop = false;
+        if (budget) {
+          const int r = compact(p, 0, st, &cv, false);
+          cv = fixed;
+          return r;
+        }
+        const bool check = ++since >= kCheckEvery || (expect > 0 && it_run >= expect - 1);
         if (check || multi) {
           const int r = compact(p, 0, st, &cv, check);
           if (r) return r;
@@ -898,21 +947,22 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         }
         return 0;
       };
+      const int nits = budget ? (sw < kBudgetSweeps ? ctl->iters[sw] : 0) : max_inner;
       bool stop = false;
       int par = 0;
       if (p->kr == 0) {  // pipelined CG
         a.par = 0;
         a.first = 1;
         STREAM_PH(PH_PCG_INIT, cv);
-        for (int it = 0; it < max_inner && !stop; it++) {
+        for (int it = 0; it < nits && !stop; it++) {
           a.par = par;
           a.first = it == 0;
           STREAM_PH(PH_PCG_ITER, cv);
           par ^= 1;
           if ((rc = next(&stop))) return rc;
         }
-      } else {  // classic CG (2) or BiCGSTAB (1): p / v buffers swap every iteration
-        for (int it = 0; it < max_inner && !stop; it++) {
+      } else {  // classic CG (2) or BiCGSTAB (1): classic CG's p buffers swap every iteration
+        for (int it = 0; it < nits && !stop; it++) {
           a.par = par;
           a.first = it == 0;
           if (p->kr == 2) {
@@ -927,13 +977,19 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
           if ((rc = next(&stop))) return rc;
         }
       }
+      if (!budget && sw < kBudgetSweeps) {
+        ctl->obs_sweeps = std::max(ctl->obs_sweeps, sw + 1);
+        ctl->obs_iters[sw] = std::max(ctl->obs_iters[sw], it_run);
+      }
       // FP64 residual over the sweep's instances (device lists, no host wait),
       // then the redo pass for columns that produced non-finite values
       cv = sweep;
       a.pass = 0;
       if ((rc = compact(p, 1, st, &cv, false))) return rc;
+      if (budget) cv = fixed;
       STREAM_PH(PH_RES, cv);
       if ((rc = compact(p, 2, st, &cv, false))) return rc;
+      if (budget) cv = fixed;
       a.pass = 1;
       STREAM_PH(PH_RES, cv);
       a.pass = 0;

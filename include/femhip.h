@@ -314,14 +314,24 @@ int fem_batch_sync(fem_batch* b);
  * sweeps and steps, its final residual and outcome bits (ms is 0: device
  * time is reported per call). */
 int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* stats);
-/* Record one fem_batch_advance(b, nsteps) of the persistent engine as a HIP
- * graph owned by the batch (the barrier-slot reset kernel and the stepping
- * kernel, no memset nodes); fem_batch_graph_launch replays it and returns
- * what fem_batch_advance would.  The state may change between replays (it is
- * read from device memory); a rebuilt launch plan (new equation parameters or
- * matrices, autotuning) makes the graph stale: the launch then returns
- * FEM_ERR_STATE and the batch must be captured again.  The streaming, plain
- * and GL-compatible engines are host-driven and are not captured. */
+/* Record one fem_batch_advance(b, nsteps) as a HIP graph owned by the batch
+ * (no memset nodes); fem_batch_graph_launch replays it and returns what
+ * fem_batch_advance would.  The state may change between replays (it is read
+ * from device memory).
+ * Persistent engine: the barrier-slot reset kernel and the stepping kernel.
+ * A rebuilt launch plan (new equation parameters or matrices, autotuning)
+ * makes the graph stale: the launch then returns FEM_ERR_STATE and the batch
+ * must be captured again.
+ * Streaming engine: every phase, reduction and compaction launch of nsteps
+ * steps, with nothing read on the host.  Each step issues the sweeps the
+ * batch's eager advances needed plus one, each of the iterations that sweep
+ * needed plus a quarter (at least 2); launches past an instance's end exit
+ * on the device.  Capture therefore needs one eager fem_batch_advance first
+ * (else FEM_ERR_STATE).  An instance that would need more iterations than
+ * recorded stops there and the replay reports FEM_NOT_CONVERGED; new
+ * equation parameters are read by the next replay; a rebuilt row-block plan
+ * (autotuning) makes the graph stale.
+ * The plain and GL-compatible engines are host-driven and are not captured. */
 int fem_batch_graph_capture(fem_batch* b, int32_t nsteps);
 int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats);
 /* Launch plan of the persistent engine (built here if it is not yet):

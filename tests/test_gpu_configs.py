@@ -153,20 +153,28 @@ def c4_cases():
     return cases
 
 
-@pytest.mark.parametrize("engine", [_lib.FEM_ENGINE_STREAM, _lib.FEM_ENGINE_PERSISTENT])
+@pytest.mark.parametrize("engine", ["stream", "stream_graph", "persistent"])
 def test_c4_batched_advdiff_16rhs(gpu, c4_cases, engine):
     """Batched advection-diffusion: the 4 smallest and 4 largest benchmark
     instances (sides 32-505) in one batch, 16 right-hand sides each,
     unnormalised velocity, dt = 1e-3, FP32 BiCGSTAB + FP64 refinement
     (tol 1e-10): every column of every instance after 2 steps against the
-    oracle."""
+    oracle.  stream_graph: one eager step, then the second as a replay of a
+    recorded streaming advance."""
     print("case C4 batched, engine", engine, [c["index"] for c in c4_cases], flush=True)
     ctxs = [FEMContext(c["mesh"], 0, velocity=c["vel"], normalize_velocity=False) for c in c4_cases]
-    b = Batch(ctxs, SolverOptions(nrhs=16, tol=1e-10, engine=engine))
+    eng = _lib.FEM_ENGINE_PERSISTENT if engine == "persistent" else _lib.FEM_ENGINE_STREAM
+    b = Batch(ctxs, SolverOptions(nrhs=16, tol=1e-10, engine=eng))
     for i, c in enumerate(c4_cases):
         b.set_equation(i, AdvectionDiffusionParameters(1e-3, c["kappa"]))
         b.set_state_unknowns(i, _lib.FEM_FIELD_U, c["F"])
-    assert b.advance(C4_STEPS) == _lib.FEM_OK
+    if engine == "stream_graph":
+        assert b.advance(C4_STEPS - 1) == _lib.FEM_OK
+        b.graph_capture(1)
+        assert b.graph_launch() == _lib.FEM_OK
+        assert b.last_stats.steps == 1
+    else:
+        assert b.advance(C4_STEPS) == _lib.FEM_OK
     for i, c in enumerate(c4_cases):
         got = b.get_state_unknowns(i).reshape(c["n"], 16)
         for j in range(16):

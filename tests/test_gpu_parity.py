@@ -846,12 +846,89 @@ def test_stream_engine_refuses_row_beyond_lds(gpu):
     assert rel(pe.get_state_unknowns(), u) < RTOL
 
 
-def test_graph_capture_needs_persistent_engine(gpu):
+def test_graph_capture_engines(gpu):
+    """The plain engine is host-driven and is not recorded; the streaming
+    engine is, once an eager advance has shown the sweeps and iterations
+    its steps need."""
     m = meshes.jittered_square(8, seed=1)
-    b = Batch([FEMContext(m, 0)], SolverOptions(engine=_lib.FEM_ENGINE_STREAM))
+    b = Batch([FEMContext(m, 0)], SolverOptions(engine=_lib.FEM_ENGINE_PLAIN))
     b.set_equation(0, HeatParameters(1e-3, 1.0))
-    with pytest.raises(_lib.FemError, match="persistent"):
+    with pytest.raises(_lib.FemError, match="host-driven"):
         b.graph_capture(2)
     with pytest.raises(_lib.FemError):
         b.graph_launch()
     b.close()
+    ctx = FEMContext(m, 0)
+    b = Batch([ctx], SolverOptions(engine=_lib.FEM_ENGINE_STREAM))
+    b.set_equation(0, HeatParameters(1e-3, 1.0))
+    b.set_state_unknowns(0, _lib.FEM_FIELD_U, np.ones(ctx.num_unknowns()))
+    with pytest.raises(_lib.FemError, match="advance the batch once first"):
+        b.graph_capture(2)
+    assert b.advance(1) == _lib.FEM_OK
+    b.graph_capture(2)
+    assert b.graph_launch() == _lib.FEM_OK
+    b.close()
+
+
+@pytest.mark.parametrize("k,krylov", [(16, _lib.FEM_KRYLOV_BICGSTAB), (1, _lib.FEM_KRYLOV_CG),
+                                      (4, _lib.FEM_KRYLOV_CG_CLASSIC)])
+def test_stream_graph_replays_match_eager(gpu, k, krylov):
+    """A recorded streaming advance (every phase, reduction and compaction of
+    two steps, with the sweeps and iterations of the eager runs plus a
+    margin, nothing read on the host) replayed several times, with eager
+    work and host reads between the replays, matches an identical batch
+    advanced eagerly bit for bit after every replay, and the oracle; state
+    writes and new equation parameters between replays are picked up."""
+    print("case stream graph replay", k, krylov, flush=True)
+    rng = np.random.default_rng(13)
+    specs = [(14, 1), (22, 2), (9, 3)]
+    vel = (0.4, -0.2, 0.0)
+    ctxs, systems, states = [], [], []
+    for side, seed in specs:
+        m = meshes.jittered_square(side, seed=seed)
+        ctxs.append(FEMContext(m, 0, velocity=vel, normalize_velocity=False))
+        systems.append(oracle.System(m, bc=0, velocity=vel, normalize=False))
+        states.append(rng.normal(size=(systems[-1].n, k)))
+    adv = krylov == _lib.FEM_KRYLOV_BICGSTAB
+    kap = 2e-2
+
+    def params(dt):
+        return AdvectionDiffusionParameters(dt, kap) if adv else HeatParameters(dt, kap)
+
+    def make():
+        b = Batch(ctxs, SolverOptions(nrhs=k, krylov=krylov, engine=_lib.FEM_ENGINE_STREAM))
+        for i in range(len(ctxs)):
+            b.set_equation(i, params(1e-3))
+            b.set_state_unknowns(i, _lib.FEM_FIELD_U, states[i])
+        return b
+
+    def oracle_run(ref, dt, n):
+        for i, s in enumerate(systems):
+            for j in range(k):
+                _, ref[i][:, j], _, _ = s.run(oracle.ADVDIFF if adv else oracle.HEAT, [dt, kap], n,
+                                              ref[i][:, j].copy(), krylov=oracle.BICGSTAB if adv else oracle.CG,
+                                              tol=1e-14)
+
+    g, e = make(), make()
+    ref = [s.copy() for s in states]
+    assert g.advance(2) == _lib.FEM_OK and e.advance(2) == _lib.FEM_OK  # sizes the recorded iterations
+    oracle_run(ref, 1e-3, 2)
+    g.graph_capture(2)
+    for rep in range(3):
+        assert g.graph_launch() == _lib.FEM_OK
+        assert g.last_stats.steps == 2
+        assert e.advance(2) == _lib.FEM_OK
+        oracle_run(ref, 1e-3, 2)
+        for i, s in enumerate(systems):
+            a, b_ = g.get_state_unknowns(i), e.get_state_unknowns(i)
+            assert np.array_equal(a, b_), (rep, i)
+            assert rel(a.reshape(s.n, k), ref[i]) < RTOL, (rep, i)
+    # a state write and new parameters between replays are read by the next one
+    for b in (g, e):
+        b.set_state_unknowns(1, _lib.FEM_FIELD_U, states[1])
+        b.set_equation(0, params(1.5e-3))
+    assert g.graph_launch() == _lib.FEM_OK and e.advance(2) == _lib.FEM_OK
+    for i in range(len(ctxs)):
+        assert np.array_equal(g.get_state_unknowns(i), e.get_state_unknowns(i)), i
+    g.close()
+    e.close()
