@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE (calibrated against a plain copy kernel) for the
 # exact command given; the summaries with their commands are in profiles/.
 PROFILED_TRAFFIC = {
-    "grayscott": {"steps": 20, "warmup": 5, "bytes_per_launch": None,
+    "grayscott": {"steps": 20, "warmup": 5, "bytes_per_launch": 1.5399e9,
                   "source": "profiles/k_step_grayscott_icosphere6.md"},
     "spmv": {"bytes_per_launch": 1.2087e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
 }
@@ -461,9 +461,10 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0, gra
     the final state), the same steps on the same engine in FP64, a sample of
     the plain FP64 engine and of the FP32 CPU port.  graph_replay (the
     batched workload on its own): the next K steps as one replay of a
-    recorded streaming advance.  The default line leaves it out, since
-    rocprofv3's counter mode crashes in the replay of a graph whose kernels
-    its kernel filter excludes."""
+    recorded streaming advance.  The default line leaves it out: rocprofv3
+    (kernel trace, and counters with a kernel filter that excludes the
+    graph's kernels) crashed in the replay of a 20-step graph, so profiled
+    commands run without one (--no-graph for the batched line)."""
     from femhip import AdvectionDiffusionParameters, Batch, FEMContext, SolverOptions, _lib
     ninst = args.instances
     t_gen = time.perf_counter()
@@ -649,7 +650,8 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0, gra
 def run_batched_c4(args):
     """The batched line on its own (C4: 200 steps by default)."""
     steps = args.steps or 200
-    r = batched_measure(args, steps, args.warmup, plain_seconds=30.0, cpu_seconds=20.0, graph_replay=True)
+    r = batched_measure(args, steps, args.warmup, plain_seconds=30.0, cpu_seconds=20.0,
+                        graph_replay=not args.no_graph)
     extra = {k: v for k, v in r.items() if k not in ("value", "unit", "steps", "warmup", "ms_per_step",
                                                       "workload", "engine")}
     return contract_line(METRIC, r["value"], r["unit"], steps, args.warmup, r["ms_per_step"],
@@ -713,6 +715,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="batched: leave out the graph replay")
     ap.add_argument("--frames", type=int, default=200,
                     help="grayscott: frames timed through the surface-space Solver path")
     ap.add_argument("--calib-copy", action="store_true",
