@@ -1387,6 +1387,22 @@ static int stream_graph_capture(fem_batch* b, int nsteps) {
     const int o = std::max(1, b->sobs_iters[std::min(sw, b->sobs_sweeps - 1)]);
     ctl.iters[sw] = std::min(max_inner, o + std::max(2, (o + 3) / 4));
   }
+  // launches the recording would hold (stream_run's schedule: per iteration
+  // the Krylov phases with their reductions and one compaction; per sweep its
+  // start, a compaction and the two residual passes; per step the right-hand
+  // side, the update and the step's end)
+  const int kr = b->splan ? b->insts[0].dev.krylov : 0;
+  const long long per_it = 2LL * (kr == 1 ? 3 : kr == 2 ? 2 : 1) + 2;
+  long long per_step = 2 + 1 + 1;
+  for (int sw = 0; sw < ctl.sweeps; sw++) per_step += 3 + (kr == 0 ? 2 : 0) + ctl.iters[sw] * per_it + 8;
+  const long long launches = 3 + (long long)nsteps * per_step;
+  if (launches > kMaxGraphLaunches) {
+    set_error("graph capture (streaming engine): " + std::to_string(nsteps) + " steps would record " +
+              std::to_string(launches) + " launches, more than " + std::to_string(kMaxGraphLaunches) +
+              "; record fewer steps (at most " + std::to_string(std::max(0LL, (kMaxGraphLaunches - 3) / per_step)) +
+              ") and replay the graph several times");
+    return FEM_ERR_INVALID;
+  }
   FEM_HIP(hipStreamSynchronize(b->stream));  // the upload of stream_setup is done before the capture
   FEM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   int turned = 0;

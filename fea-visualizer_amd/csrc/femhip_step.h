@@ -45,6 +45,11 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
 // `sweeps` sweeps of iters[sw] iterations over a grid of `grid` workgroups,
 // and the launches past an instance's end exit at once on the device.
 constexpr int kBudgetSweeps = 8;
+// Most kernel launches one streaming graph records.  A 200-step recording of
+// the 256-instance batch (~250,000 launches) ended the process with SIGSEGV
+// in graph capture or replay, where 20 steps (~25,000) ran; longer advances
+// replay a shorter recording several times.
+constexpr long long kMaxGraphLaunches = 8192;
 struct StreamCtl {
   bool budget = false;
   int sweeps = 0, grid = 0;

@@ -330,7 +330,9 @@ int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* sta
  * (else FEM_ERR_STATE).  An instance that would need more iterations than
  * recorded stops there and the replay reports FEM_NOT_CONVERGED; new
  * equation parameters are read by the next replay; a rebuilt row-block plan
- * (autotuning) makes the graph stale.
+ * (autotuning) makes the graph stale.  A recording holds at most 8192
+ * launches (FEM_ERR_INVALID beyond, naming the steps that fit): longer
+ * advances replay a shorter recording several times.
  * The plain and GL-compatible engines are host-driven and are not captured. */
 int fem_batch_graph_capture(fem_batch* b, int32_t nsteps);
 int fem_batch_graph_launch(fem_batch* b, fem_step_stats* stats);

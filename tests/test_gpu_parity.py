@@ -867,6 +867,8 @@ def test_graph_capture_engines(gpu):
     assert b.advance(1) == _lib.FEM_OK
     b.graph_capture(2)
     assert b.graph_launch() == _lib.FEM_OK
+    with pytest.raises(_lib.FemError, match="launches"):  # refused on the host, nothing recorded
+        b.graph_capture(100000)
     b.close()
 
 
