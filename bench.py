@@ -97,6 +97,12 @@ def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False, fused=
     return iters * it + sweeps * sw + steps * st
 
 
+def stats_copy(st):
+    """A copy of a solver's last_stats: later advances (accuracy checks)
+    overwrite the object in place."""
+    return type(st).from_buffer_copy(st)
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -282,7 +288,7 @@ def run_heat_c1(args):
     status = sol.advance(steps)
     sol.sync()
     wall = time.perf_counter() - t
-    st = sol.last_stats
+    st = stats_copy(sol.last_stats)
     ach = step_bytes(n, nnz, 1, 1, st.iterations, st.sweeps, steps, ti=8, fused=True) / (st.ms * 1e-3) / 1e9
     import oracle
     so = oracle.System(mesh, bc=0)
@@ -355,7 +361,7 @@ def run_wave_c3(args):
     status = sol.advance(steps)
     sol.sync()
     wall = time.perf_counter() - t
-    st = sol.last_stats
+    st = stats_copy(sol.last_stats)
     ach = step_bytes(n, nnz, 1, 1, st.iterations, st.sweeps, steps) / (st.ms * 1e-3) / 1e9
     ref = fresh(precision=_lib.FEM_PREC_FP64, tol=1e-13)
     ref.advance(steps)
@@ -511,7 +517,7 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0, gra
     log("batched: autotuned", tuned)
     engine = "stream" if b.options.engine == _lib.FEM_ENGINE_STREAM else "persistent"
     status, wall = timed(b)
-    st = b.last_stats
+    st = stats_copy(b.last_stats)
     log("batched: timed run", round(wall, 3), "s")
     dof = 16 * tot_n
     # algorithmic bytes instance by instance: larger meshes iterate more
@@ -829,7 +835,7 @@ def grayscott_line(args):
         tuned = sol.autotune(20)  # untimed; the state is restored
     status, wall = timed(sol)
     log("grayscott: timed run", wall)
-    st = sol.last_stats
+    st = stats_copy(sol.last_stats)
     kernel_ms = st.ms
     value = 2 * n * K / wall
     out = contract_line(
