@@ -21,6 +21,7 @@ Prints one JSON line.  Extra keys:
                     surface-space Solver boundary (set_state, 1 step, get_state)
   spmv, adjoint     the SpMV bandwidth and adjoint/forward cost legs
   batched           the 256-mesh x 16-RHS batched line, K steps after W
+  wave, heat        the Newmark 1024^2 and heat 64^2 lines, K steps each
   cpu_baseline      the FP32 port of the reference CPU path, one core
 """
 from __future__ import annotations
@@ -737,6 +738,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="batched: leave out the graph replay")
+    ap.add_argument("--no-wave", action="store_true", help="default line: leave out the Newmark wave leg")
+    ap.add_argument("--no-heat", action="store_true", help="default line: leave out the heat leg")
     ap.add_argument("--frames", type=int, default=200,
                     help="grayscott: frames timed through the surface-space Solver path")
     ap.add_argument("--calib-copy", action="store_true",
@@ -778,6 +781,14 @@ def main():
     if not args.no_batched:
         log("batched")
         out["batched"] = batched_measure(args, args.steps if args.steps <= 200 else 200, args.warmup)
+    # the other single-instance benchmark workloads, K steps each (at most 200)
+    short = argparse.Namespace(**{**vars(args), "steps": min(args.steps, 200)})
+    if not args.no_wave:
+        log("wave")
+        out["wave"] = run_wave_c3(short)
+    if not args.no_heat:
+        log("heat")
+        out["heat"] = run_heat_c1(short)
     if not args.no_cpu and args.cpu_steps > 0:
         log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(out.pop("_mesh"), out.pop("_u0"), out.pop("_v0"),
