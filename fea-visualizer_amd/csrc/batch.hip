@@ -1023,6 +1023,13 @@ int batch_prepare(fem_batch* b, bool need_plan) {
       if (!rc && need_plan && !h.d_hvec) h.d_hvec = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
       if (rc) return rc;
     }
+    // Newmark on the streaming engine: each update forms the next step's
+    // displacement predictor (Inst::hvec)
+    if (h.eq == FEM_WAVE_NEWMARK && !need_plan && !h.d_hvec) {
+      int rc = 0;
+      h.d_hvec = dalloc<double>((size_t)h.ctx->n * h.k, &rc);
+      if (rc) return rc;
+    }
     if (h.dirty || h.formed_version != h.ctx->version) {
       int rc = form_inst(b, h);
       if (rc) return rc;

@@ -118,7 +118,8 @@ struct Inst {
   double* xnext;         // [n][k] next initial guess formed with the update (persistent kernel)
   double* hpre;          // [n][k] history part of the next initial residual
   double* hvec;          // [n][k] history part of the next initial guess, formed with the update:
-                         //   phase_res gathers it instead of the history levels
+                         //   phase_res gathers it instead of the history levels; Newmark on the
+                         //   streaming engine: the next u + dt v + (1/2 - beta) dt^2 a
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

@@ -174,6 +174,51 @@ __device__ void stream_update_first(const Inst& I, int row_lo, int row_hi, const
   if (f) atomicOr(flags, f);
 }
 
+// Newmark update (phase_update's arithmetic, step_kernel.h).  more: another
+// step follows in this launch, so also its displacement predictor
+// u + dt v + (1/2 - beta) dt^2 a into hvec and, lvn >= 0, its guess of the
+// acceleration (order lvn) into x: the next right-hand side (vphase_rhs)
+// then gathers two vectors instead of five.
+template <int K, typename TI>
+__device__ void stream_update_newmark(const Inst& I, int row_lo, int row_hi, const Shared& sh, int* flags, int step,
+                                      int lvn, bool more) {
+  constexpr int RPP = kBlock / K;
+  const int jr = threadIdx.x % K;
+  TI GA* xi = gp((TI*)I.xi);
+  const double dt = I.dt, be = I.p2, ga = I.p3;
+  const double hh = (0.5 - be) * dt * dt;
+  int f = 0;
+  double* traj = nullptr;  // trajectory record of this step, if any
+  if (I.traj && (step + 1) % I.traj_every == 0)
+    traj = I.traj + (size_t)((step + 1) / I.traj_every - 1) * I.n * K;
+  for (int row = row_lo + (int)threadIdx.x / K; row < row_hi; row += RPP) {
+    const size_t o = (size_t)row * K + jr;
+    double xf = gp(I.x)[o] + (double)xi[o];
+    const double uo = gp(I.u)[o], ao = gp(I.a)[o], vo = gp(I.v)[o];
+    const double ho = I.extrap >= 2 ? gp(I.h3)[o] : 0.0;
+    if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
+    const double ut = uo + dt * vo + hh * ao;
+    const double un = ut + be * dt * dt * xf;
+    const double vn = vo + dt * ((1.0 - ga) * ao + ga * xf);
+    gp(I.u)[o] = un;
+    gp(I.v)[o] = vn;
+    if (I.extrap >= 2) gp(I.h4)[o] = ho;  // acceleration history for the predictor
+    if (I.extrap >= 1) gp(I.h3)[o] = ao;
+    gp(I.a)[o] = xf;
+    if (more) {
+      gp(I.hvec)[o] = un + dt * vn + hh * xf;
+      if (lvn >= 0) gp(I.x)[o] = lvn == 2 ? 3.0 * (xf - ao) + ho : lvn == 1 ? 2.0 * xf - ao : xf;
+    } else {
+      gp(I.x)[o] = xf;
+      xi[o] = (TI)0;
+    }
+    if (traj) gp(traj)[o] = un;
+    if (un > 1e4 || vn > 1e4) f |= ST_UNSTABLE;
+    if (!isfinite(un) || !isfinite(vn)) f |= ST_NONFINITE;
+  }
+  if (f) atomicOr(flags, f);
+}
+
 // V adjacent columns per lane (stream_vec.h); g: the staged pattern of the
 // gathering Krylov phases.
 template <int K, int V, typename TI, int PH>
@@ -270,6 +315,8 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
       if (first_order_eq(I.eq))  // the guess goes to x where vphase_rhs gathers it
         stream_update_first<K, TI>(I, row_lo, row_hi, sh, I.status, a.step, lvn,
                                    vec_layout<K>() && lvn > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF));
+      else if (vec_layout<K>() && I.eq == EQ_NEWMARK && I.hvec)
+        stream_update_newmark<K, TI>(I, row_lo, row_hi, sh, I.status, a.step, lvn, a.more != 0);
       else
         phase_update<K, TI, kPlain>(I, row_lo, row_hi, sh, I.status, a.step, lvn);
     } else if constexpr (vec_layout<K>() && staged_phase<PH>()) {
@@ -330,11 +377,15 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
     scratch[c * nv + q] = s;
   }
   __syncthreads();
-  if (threadIdx.x < nv) {
-    double t = 0.0;
-    for (int cc = 0; cc < chunks; cc++) t += scratch[cc * nv + threadIdx.x];
-    red[threadIdx.x] = t;
+  // pairwise tree over the chunks in a fixed order (deterministic): a chain
+  // of up to kRedBlock dependent adds would dominate the reduction
+  int span = 1;
+  while (span < chunks) span <<= 1;
+  for (int h = span >> 1; h > 0; h >>= 1) {
+    if (c < h && c + h < chunks) scratch[c * nv + q] += scratch[(c + h) * nv + q];
+    __syncthreads();
   }
+  if (threadIdx.x < nv) red[threadIdx.x] = scratch[threadIdx.x];
   __syncthreads();
 }
 
@@ -694,6 +745,10 @@ const void* pick_reduce(int k, int ph) {
 
 bool stream_vec_layout(int k) { return k == 1 || k >= 4; }
 
+// Pinned slots for the host's reads of the list counts: a read is queued in
+// one slot while the previous one is still being waited for.
+constexpr int kCountSlots = 2;
+
 struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
   int stage_rows = 0, stage_nnz = 0;  // largest row block: rows, nonzeros (up to kStageCap)
@@ -708,7 +763,8 @@ struct StreamPlan {
   int* d_act_base = nullptr;  // [ninst]
   int* d_act_blk = nullptr;   // [nblocks]
   int* d_cnt = nullptr;       // {active instances, active blocks}
-  int* h_cnt = nullptr;       // pinned copy
+  int* h_cnt = nullptr;       // pinned copies: kCountSlots x {instances, blocks}
+  hipEvent_t ev[kCountSlots] = {};  // marks each slot's copy
 };
 
 void stream_plan_free(StreamPlan* p) {
@@ -718,6 +774,8 @@ void stream_plan_free(StreamPlan* p) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->h_cnt) (void)hipHostFree(p->h_cnt);
+  for (hipEvent_t e : p->ev)
+    if (e) (void)hipEventDestroy(e);
   delete p;
 }
 
@@ -779,8 +837,12 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_cnt = dalloc<int>(2, &rc);
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&p->h_cnt, 2 * sizeof(int), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&p->h_cnt, 2 * kCountSlots * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
+    for (int q = 0; q < kCountSlots && !rc; q++) {
+      e = hipEventCreateWithFlags(&p->ev[q], hipEventDisableTiming);
+      if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate");
+    }
   }
   if (rc) {
     stream_plan_free(p);
@@ -843,24 +905,32 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
 }
 
 // Compact the instances and row blocks that satisfy `pred` into the device
-// lists (k_compact, k_expand).  With `read`, wait for the counts and make them
-// the launch sizes: nab == 0 means none is left.
-int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, bool read) {
+// lists (k_compact, k_expand).  post >= 0: also queue the copy of the lists'
+// counts into that pinned slot and mark it with the slot's event (read back
+// by wait_counts).  The launch sizes in *cv are left as they are: a superset.
+int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, int post = -1) {
   hipLaunchKernelGGL(k_compact, dim3(1), dim3(kBlock), 0, st, p->d_ss, p->d_inst_blk, p->ninst, pred, p->d_act_inst,
                      p->d_act_base, p->d_cnt);
   FEM_HIP(hipGetLastError());
-  if (read) FEM_HIP(hipMemcpyAsync(p->h_cnt, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  if (post >= 0) {
+    FEM_HIP(hipMemcpyAsync(p->h_cnt + 2 * post, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    FEM_HIP(hipEventRecord(p->ev[post], st));
+  }
   if (p->nblocks > 0) {  // queued before the host waits, so it overlaps the read-back
     hipLaunchKernelGGL(k_expand, dim3(div_up(p->nblocks, kBlock)), dim3(kBlock), 0, st, p->d_blk_inst, p->d_inst_blk,
                        p->d_act_base, p->nblocks, p->d_act_blk);
     FEM_HIP(hipGetLastError());
   }
   cv->all = false;
-  if (read) {
-    FEM_HIP(hipStreamSynchronize(st));
-    cv->nact = p->h_cnt[0];
-    cv->nab = p->h_cnt[1];
-  }
+  return 0;
+}
+
+// Wait for the counts queued in `slot`: nab == 0 means no block is left.
+int wait_counts(const StreamPlan* p, int slot, Cover* cv) {
+  FEM_HIP(hipEventSynchronize(p->ev[slot]));
+  cv->all = false;
+  cv->nact = p->h_cnt[2 * slot];
+  cv->nab = p->h_cnt[2 * slot + 1];
   return 0;
 }
 
@@ -904,6 +974,13 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     for (int& v : ctl->obs_iters) v = 0;
   }
   const int nsweeps = budget ? ctl->sweeps : max_sweeps;
+  // Eager runs read the lists' counts on the host, at most one read in
+  // flight: each read is queued in a pinned slot and waited for only after
+  // the next launches are queued, so the device does not drain while the
+  // host waits.  The launches in between cover a superset (instances only
+  // leave the lists) and blocks past the device's counts exit at once.
+  int slot = 0;
+  const Cover every{false, p->ninst, p->nblocks};
 #define STREAM_PH(ph, cover)                                     \
   do {                                                           \
     if ((rc = launch_phase(p, a, (ph), (cover), st))) return rc; \
@@ -914,36 +991,71 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     a.first = 1;
     a.pass = 0;
     STREAM_PH(PH_RHS, full);
+    Cover prev_sweep = every;  // a superset of the next sweep's instances
     for (int sw = 0; sw < nsweeps; sw++) {
       hipLaunchKernelGGL(k_sweep_begin, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts,
                          p->ninst, p->k, p->fp32);
       FEM_HIP(hipGetLastError());
-      if ((rc = compact(p, 0, st, &cv, !budget))) return rc;  // instances that sweep
-      if (budget) cv = fixed;
-      else if (cv.nab == 0) break;
-      const Cover sweep = cv;  // covers every instance of this sweep
-      // one iteration done: refresh the lists (several instances) and, every
-      // kCheckEvery-th time, read whether any instance still iterates
-      // (budgeted: refresh the lists only)
-      int since = 0, it_run = 0;
+      // instances that sweep; eager: their counts arrive after the sweep's
+      // first launches, which run over the previous sweep's cover meanwhile
+      int sweep_slot = -1;
+      if (budget) {
+        if ((rc = compact(p, 0, st, &cv))) return rc;
+        cv = fixed;
+      } else {
+        slot ^= 1;
+        sweep_slot = slot;
+        if ((rc = compact(p, 0, st, &cv, sweep_slot))) return rc;
+        cv = prev_sweep;
+      }
+      Cover sweep = cv;  // covers every instance of this sweep
+      bool none = false;
+      auto settle = [&]() -> int {
+        if (sweep_slot < 0) return 0;
+        const int r = wait_counts(p, sweep_slot, &cv);
+        sweep_slot = -1;
+        sweep = cv;
+        none = cv.nab == 0;
+        return r;
+      };
+      // one iteration queued: refresh the lists (several instances), take
+      // the counts read after the previous iteration, and every
+      // kCheckEvery-th time queue another read (budgeted: refresh only)
+      int since = 0, it_run = 0, pend = -1, pend_it = 0, need = 0;
       const int expect = sw < (int)prev_its.size() ? prev_its[sw] : 0;
       auto next = [&](bool* stop) -> int {
         ++it_run;
         *stop = false;
         if (budget) {
-          const int r = compact(p, 0, st, &cv, false);
+          const int r = compact(p, 0, st, &cv);
           cv = fixed;
           return r;
         }
         const bool check = ++since >= kCheckEvery || (expect > 0 && it_run >= expect - 1);
-        if (check || multi) {
-          const int r = compact(p, 0, st, &cv, check);
-          if (r) return r;
-        }
+        int post = -1;
         if (check) {
           since = 0;
-          *stop = cv.nab == 0;
-          if (*stop && sw < (int)prev_its.size()) prev_its[sw] = it_run;
+          slot ^= 1;
+          post = slot;
+        }
+        int r = 0;
+        if ((check || multi) && (r = compact(p, 0, st, &cv, post))) return r;
+        if (pend >= 0) {
+          Cover got{};
+          if ((r = wait_counts(p, pend, &got))) return r;
+          pend = -1;
+          if (got.nab == 0) {  // done after iteration pend_it: this one is empty
+            *stop = true;
+            need = pend_it;
+            if (sw < (int)prev_its.size()) prev_its[sw] = pend_it;
+            return 0;
+          }
+          cv.nact = got.nact;
+          cv.nab = got.nab;
+        }
+        if (post >= 0) {
+          pend = post;
+          pend_it = it_run;
         }
         return 0;
       };
@@ -954,7 +1066,8 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         a.par = 0;
         a.first = 1;
         STREAM_PH(PH_PCG_INIT, cv);
-        for (int it = 0; it < nits && !stop; it++) {
+        if ((rc = settle())) return rc;
+        for (int it = 0; it < nits && !stop && !none; it++) {
           a.par = par;
           a.first = it == 0;
           STREAM_PH(PH_PCG_ITER, cv);
@@ -962,7 +1075,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
           if ((rc = next(&stop))) return rc;
         }
       } else {  // classic CG (2) or BiCGSTAB (1): classic CG's p buffers swap every iteration
-        for (int it = 0; it < nits && !stop; it++) {
+        for (int it = 0; it < nits && !stop && !none; it++) {
           a.par = par;
           a.first = it == 0;
           if (p->kr == 2) {
@@ -974,25 +1087,33 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
             STREAM_PH(PH_BI_X, cv);
           }
           par ^= 1;
+          if (it == 0) {
+            if ((rc = settle())) return rc;
+            if (none) break;
+          }
           if ((rc = next(&stop))) return rc;
         }
+        if ((rc = settle())) return rc;  // no iteration ran (nits == 0)
       }
+      if (none) break;  // no instance swept: its launches were empty
+      if (!stop) need = it_run;
       if (!budget && sw < kBudgetSweeps) {
         ctl->obs_sweeps = std::max(ctl->obs_sweeps, sw + 1);
-        ctl->obs_iters[sw] = std::max(ctl->obs_iters[sw], it_run);
+        ctl->obs_iters[sw] = std::max(ctl->obs_iters[sw], need);
       }
       // FP64 residual over the sweep's instances (device lists, no host wait),
       // then the redo pass for columns that produced non-finite values
       cv = sweep;
       a.pass = 0;
-      if ((rc = compact(p, 1, st, &cv, false))) return rc;
+      if ((rc = compact(p, 1, st, &cv))) return rc;
       if (budget) cv = fixed;
       STREAM_PH(PH_RES, cv);
-      if ((rc = compact(p, 2, st, &cv, false))) return rc;
+      if ((rc = compact(p, 2, st, &cv))) return rc;
       if (budget) cv = fixed;
       a.pass = 1;
       STREAM_PH(PH_RES, cv);
       a.pass = 0;
+      prev_sweep = sweep;
     }
     a.more = step + 1 < nsteps;
     STREAM_PH(PH_UPD, full);

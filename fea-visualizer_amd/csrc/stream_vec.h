@@ -158,8 +158,11 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
   const int lv = warm ? min(I.extrap, I.hist + step) : 0;  // predictor order this step
   const bool ex = lv > 0;
   // after the first step of a launch the update wrote this step's guess into
-  // x (stream_update_first, stream.hip)
-  const bool pre = ex && step > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF);
+  // x (stream_update_first, stream_update_newmark, stream.hip) and, for
+  // Newmark, the displacement predictor u + dt v + (1/2 - beta) dt^2 a into
+  // hvec
+  const bool nm = I.eq == EQ_NEWMARK && step > 0 && I.hvec;
+  const bool pre = (ex && step > 0 && (I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF)) || (nm && warm);
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* M64 = gp(I.M64);
@@ -225,6 +228,16 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
         x0 = warm ? (ex ? 2.0 * vo - vld<V>(I.a + o) : vo) : zd;
       } break;
       case EQ_NEWMARK: {  // b = -c^2 K (u + dt v + (1/2 - beta) dt^2 a)
+        if (nm) {  // gather the predictor and the guess the previous update formed
+          const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
+            const size_t oc = (size_t)ci[s] * K + j0;
+            return P2<DV>{K64[s] * vld<V>(I.hvec + oc), warm ? S64[s] * vld<V>(I.x + oc) : zd};
+          });
+          b = -I.p1 * I.p1 * g.a;
+          sx = g.b;
+          x0 = warm ? vld<V>(I.x + o) : zd;
+          break;
+        }
         const double h = (0.5 - I.p2) * I.dt * I.dt;
         const P2<DV> g = vgather<U>(lo, hi, P2<DV>{zd, zd}, [&](int s) {
           const size_t oc = (size_t)ci[s] * K + j0;
