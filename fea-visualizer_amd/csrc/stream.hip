@@ -13,8 +13,9 @@
 // (deterministic) and applies the same scalar recurrences as the persistent
 // drivers to a per-instance scalar record in device memory, which the next
 // phase reads.  All instances iterate in lockstep; instances whose sweep or
-// solve has finished skip the launches.  The host reads one "all finished"
-// flag per iteration to stop the loop.
+// solve has finished skip the launches.  The host reads the counts of the
+// instances still iterating every few iterations, one read in flight behind
+// the queued launches, to stop the loop.
 //
 // For one or two right-hand sides the phase bodies are the persistent
 // kernel's own (plain exchange mode); for four or more, each lane owns 16
