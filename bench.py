@@ -346,11 +346,12 @@ def run_wave_c3(args):
         sol.set_state_unknowns(_lib.FEM_FIELD_U, u0)
         sol.init_newmark_acceleration()
         return sol
+    itol = args.inner_tol or 1e-6  # one FP32 sweep plus one refinement per step (1e-4: 2.8-3.0 sweeps)
     if args.engine == "stream":
-        sol = fresh(tol=1e-10, engine=_lib.FEM_ENGINE_STREAM)
+        sol = fresh(tol=1e-10, inner_tol=itol, engine=_lib.FEM_ENGINE_STREAM)
         tuned = None
     else:
-        sol = fresh(tol=1e-10)
+        sol = fresh(tol=1e-10, inner_tol=itol)
         tuned = sol.autotune(5) if not args.no_autotune else None
     a0 = sol.get_state_unknowns(_lib.FEM_FIELD_A)
     sol.advance(args.warmup)
@@ -385,7 +386,7 @@ def run_wave_c3(args):
                                                           if sol.options.engine == _lib.FEM_ENGINE_STREAM
                                                           else "persistent cooperative kernel")},
                         status=int(status),
-                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "autotune": tuned,
+                        solver={"iterations": int(st.iterations), "sweeps": int(st.sweeps), "inner_tol": itol, "autotune": tuned,
                                 "kernel_ms": st.ms, "iters_per_step": st.iterations / steps},
                         roofline={"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": ach / HBM_PEAK_GBS, "traffic": None,
