@@ -173,6 +173,152 @@ __global__ void __launch_bounds__(kBlock) k_spmv_stream(int n, long long nnz, co
   else *(vec*)(y + (size_t)row * K + c) = acc;
 }
 
+// Pipelined row-block stream: the stream schedule's per-block work, with a
+// persistent grid walking the row blocks (block b = blockIdx.x + j gridDim.x,
+// gridDim.x a multiple of 8, so every block stays on the XCD the plain
+// schedule deals it to).  A block's set-up is two dependent round trips, its
+// row pointers and then its column/value stream, which the plain schedule
+// waits for before every block's gathers.  Here, while a workgroup gathers
+// block j out of LDS, the stream of block j + 1 is in flight into registers
+// and the row pointers of block j + 2 behind it; block j + 1's stream goes to
+// the other LDS buffer after the gathers, and one barrier per block follows.
+template <typename T, int K, bool NT>
+__global__ void __launch_bounds__(kBlock) k_spmv_pipe(int n, long long nnz, int nblk, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci, const T* __restrict__ val,
+                                                      const T* __restrict__ x, T* __restrict__ y) {
+  using SS = StreamShape<T, K>;
+  constexpr int V = SS::V, LPR = SS::LPR, R = SS::R;
+  constexpr int CAP = R * kCapRow;                       // multiple of 4
+  constexpr int NCH = (CAP / 4 + kBlock - 1) / kBlock;   // 4-entry chunks per thread
+  typedef T vec __attribute__((ext_vector_type(V)));
+  typedef T v4t __attribute__((ext_vector_type(4)));
+  __shared__ int s_ci[2][CAP];
+  __shared__ T s_val[2][CAP];
+  const int tid = (int)threadIdx.x;
+  const int c = (tid % LPR) * V;
+  struct Ptrs {
+    int lo, hi, rlo, rhi;
+  };
+  struct Chunks {
+    v4i ci[NCH];
+    v4t va[NCH];
+  };
+  // round trip 1: the block's nonzero range and the lane's row
+  auto load_ptrs = [&](int b) {
+    Ptrs P{0, 0, 0, 0};
+    if (b < nblk) {
+      const int row0 = b * R, row1 = min(n, row0 + R);
+      const int row = row0 + tid / LPR;
+      P.lo = rp[row0];
+      P.hi = rp[row1];
+      if (row < n) {
+        P.rlo = rp[row];
+        P.rhi = rp[row + 1];
+      }
+    }
+    return P;
+  };
+  auto staged = [&](const Ptrs& P) { return P.hi - (P.lo & ~3) <= CAP; };
+  // round trip 2: the block's column/value stream, 16-byte loads from a
+  // 16-byte aligned start (element loads at the arrays' tail)
+  auto load_chunks = [&](const Ptrs& P, bool valid, Chunks& C) {
+    if (!valid || !staged(P)) return;
+    const int base = P.lo & ~3;
+    const int nch = (P.hi - base + 3) >> 2;
+#pragma unroll
+    for (int u = 0; u < NCH; u++) {
+      const int q = tid + u * kBlock;
+      if (q >= nch) continue;
+      const long long e0 = (long long)base + 4 * q;
+      if (e0 + 3 < nnz) {
+        C.ci[u] = load16<NT>((const v4i*)(ci + e0));
+        C.va[u] = load16<NT>((const v4t*)(val + e0));
+      } else {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+          C.ci[u][w] = e0 + w < nnz ? ci[e0 + w] : 0;
+          C.va[u][w] = e0 + w < nnz ? val[e0 + w] : (T)0;
+        }
+      }
+    }
+  };
+  auto store_chunks = [&](const Ptrs& P, bool valid, const Chunks& C, int buf) {
+    if (!valid || !staged(P)) return;
+    const int nch = (P.hi - (P.lo & ~3) + 3) >> 2;
+#pragma unroll
+    for (int u = 0; u < NCH; u++) {
+      const int q = tid + u * kBlock;
+      if (q >= nch) continue;
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        s_ci[buf][4 * q + w] = C.ci[u][w];
+        s_val[buf][4 * q + w] = C.va[u][w];
+      }
+    }
+  };
+  auto walk = [&](const int* cs, const T* vs, int off, int rlo, int rhi) {
+    vec acc = (vec)(T)0;
+    for (int s0 = rlo; s0 < rhi; s0 += 9) {
+      vec t[9];
+#pragma unroll
+      for (int u = 0; u < 9; u++) {
+        const int q = min(s0 + u, rhi - 1) - off;
+        t[u] = vs[q] * *(const vec*)(x + (size_t)cs[q] * K + c);
+      }
+#pragma unroll
+      for (int u = 0; u < 9; u++)
+        if (s0 + u < rhi) acc += t[u];
+    }
+    return acc;
+  };
+  const int G = (int)gridDim.x;
+  int b = (int)blockIdx.x;
+  if (b >= nblk) return;  // the whole workgroup
+  Ptrs P0 = load_ptrs(b);
+  Chunks C0;
+  load_chunks(P0, true, C0);
+  Ptrs P1 = load_ptrs(b + G);
+  store_chunks(P0, true, C0, 0);
+  __syncthreads();
+  int buf = 0;
+  for (; b < nblk; b += G) {
+    const bool has1 = b + G < nblk;
+    const Ptrs P2 = load_ptrs(b + 2 * G);  // round trip 1 of block j + 2
+    Chunks C1;
+    load_chunks(P1, has1, C1);             // round trip 2 of block j + 1
+    const int row = b * R + tid / LPR;
+    if (row < n) {
+      const vec acc = staged(P0) ? walk(s_ci[buf], s_val[buf], P0.lo & ~3, P0.rlo, P0.rhi)
+                                 : walk(ci, val, 0, P0.rlo, P0.rhi);
+      if constexpr (NT) __builtin_nontemporal_store(acc, (vec*)(y + (size_t)row * K + c));
+      else *(vec*)(y + (size_t)row * K + c) = acc;
+    }
+    store_chunks(P1, has1, C1, buf ^ 1);
+    __syncthreads();
+    P0 = P1;
+    P1 = P2;
+    buf ^= 1;
+  }
+}
+
+// Workgroups of k_spmv_pipe's grid: as many as fit on the device at once,
+// a multiple of 8 (one share per XCD), at most one per row block.
+template <typename T, int K, bool NT>
+int pipe_grid(int nblk, int* grid) {
+  static int occ = 0;  // the same for every device of the build's target
+  if (occ == 0) {
+    int o = 0;
+    FEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)k_spmv_pipe<T, K, NT>, kBlock, 0));
+    occ = std::max(1, o);
+  }
+  int dev = 0, cus = 0;
+  FEM_HIP(hipGetDevice(&dev));
+  FEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int full = std::max(8, (cus * occ) & ~7);
+  *grid = std::max(1, std::min(nblk, full));
+  return 0;
+}
+
 template <typename T, int K>
 const void* pick_g(int g) {
   switch (g) {
@@ -183,6 +329,8 @@ const void* pick_g(int g) {
     case 16: return (const void*)k_spmv<T, K, 16>;
     case FEM_SPMV_STREAM: return (const void*)k_spmv_stream<T, K, false>;
     case FEM_SPMV_STREAM_NT: return (const void*)k_spmv_stream<T, K, true>;
+    case FEM_SPMV_PIPE: return (const void*)k_spmv_pipe<T, K, false>;
+    case FEM_SPMV_PIPE_NT: return (const void*)k_spmv_pipe<T, K, true>;
     default: return nullptr;
   }
 }
@@ -210,6 +358,16 @@ int stream_rows(int k) {
 
 int default_lanes(int k) { return k >= 16 ? 16 : FEM_SPMV_STREAM_NT; }
 
+template <typename T>
+int pipe_grid_k(int k, bool nt, int nblk, int* grid) {
+  switch (k) {
+    case 1: return nt ? pipe_grid<T, 1, true>(nblk, grid) : pipe_grid<T, 1, false>(nblk, grid);
+    case 2: return nt ? pipe_grid<T, 2, true>(nblk, grid) : pipe_grid<T, 2, false>(nblk, grid);
+    case 4: return nt ? pipe_grid<T, 4, true>(nblk, grid) : pipe_grid<T, 4, false>(nblk, grid);
+    default: return nt ? pipe_grid<T, 16, true>(nblk, grid) : pipe_grid<T, 16, false>(nblk, grid);
+  }
+}
+
 int launch(fem_operator* op, int k, int g, const void* x, void* y) {
   const void* fn = op->precision == FEM_PREC_FP32_IR ? pick_k<float>(k, g) : pick_k<double>(k, g);
   if (!fn) {
@@ -223,10 +381,20 @@ int launch(fem_operator* op, int k, int g, const void* x, void* y) {
   const int* ci = op->ctx->d_ci;
   const void* val = op->d_val;
   if (g >= FEM_SPMV_STREAM) {
-    const int rows = op->precision == FEM_PREC_FP32_IR ? stream_rows<float>(k) : stream_rows<double>(k);
-    const unsigned grid = div_up(op->n, rows);
+    const bool f32 = op->precision == FEM_PREC_FP32_IR;
+    const int rows = f32 ? stream_rows<float>(k) : stream_rows<double>(k);
+    int nblk = (int)div_up(op->n, rows);
+    if (g == FEM_SPMV_PIPE || g == FEM_SPMV_PIPE_NT) {
+      int grid = 0;
+      const bool nt = g == FEM_SPMV_PIPE_NT;
+      const int rc = f32 ? pipe_grid_k<float>(k, nt, nblk, &grid) : pipe_grid_k<double>(k, nt, nblk, &grid);
+      if (rc) return rc;
+      void* args[] = {(void*)&n, (void*)&nnz, (void*)&nblk, (void*)&rp, (void*)&ci, (void*)&val, (void*)&x, (void*)&y};
+      FEM_HIP(hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, op->stream));
+      return 0;
+    }
     void* args[] = {(void*)&n, (void*)&nnz, (void*)&rp, (void*)&ci, (void*)&val, (void*)&x, (void*)&y};
-    FEM_HIP(hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, 0, op->stream));
+    FEM_HIP(hipLaunchKernel(fn, dim3(nblk), dim3(kBlock), args, 0, op->stream));
     return 0;
   }
   const unsigned grid = div_up((long long)op->n * g, kBlock);
@@ -236,7 +404,8 @@ int launch(fem_operator* op, int k, int g, const void* x, void* y) {
 }
 
 bool valid_sched(int g) {
-  return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == FEM_SPMV_STREAM || g == FEM_SPMV_STREAM_NT;
+  return g == 1 || g == 2 || g == 4 || g == 8 || g == 16 || g == FEM_SPMV_STREAM || g == FEM_SPMV_STREAM_NT ||
+         g == FEM_SPMV_PIPE || g == FEM_SPMV_PIPE_NT;
 }
 
 bool valid_k(int k) { return k == 1 || k == 2 || k == 4 || k == 16; }
@@ -386,7 +555,8 @@ int operator_autotune(fem_operator* op, int k, const void* x, void* y, int reps,
   }
   double best = 1e300;
   int bg = default_lanes(k);
-  for (int g : {1, 2, 4, 8, 16, (int)FEM_SPMV_STREAM, (int)FEM_SPMV_STREAM_NT}) {
+  for (int g : {1, 2, 4, 8, 16, (int)FEM_SPMV_STREAM, (int)FEM_SPMV_STREAM_NT, (int)FEM_SPMV_PIPE,
+                (int)FEM_SPMV_PIPE_NT}) {
     if (g < k) continue;
     double t = 0.0;
     const int rc = time_lanes(op, k, g, x, y, reps, &t);

@@ -64,7 +64,9 @@ FEM_MESH_CLOSED = 1
 FEM_MESH_MIRRORED = 2
 FEM_SPMV_STREAM = 32
 FEM_SPMV_STREAM_NT = 33
-SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT)
+FEM_SPMV_PIPE = 34
+FEM_SPMV_PIPE_NT = 35
+SPMV_SCHEDULES = (1, 2, 4, 8, 16, FEM_SPMV_STREAM, FEM_SPMV_STREAM_NT, FEM_SPMV_PIPE, FEM_SPMV_PIPE_NT)
 
 
 class fem_mesh(C.Structure):

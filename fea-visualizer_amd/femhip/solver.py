@@ -439,7 +439,9 @@ class Batch:
         return st
 
     def graph_capture(self, nsteps: int):
-        """Record one advance(nsteps) of the persistent engine as a HIP graph."""
+        """Record one advance(nsteps) as a HIP graph (persistent or streaming
+        engine; a streaming batch needs one eager advance first, see
+        fem_batch_graph_capture)."""
         L.check(self._lib.fem_batch_graph_capture(self.handle, int(nsteps)))
 
     def graph_launch(self) -> int:

@@ -380,8 +380,10 @@ int fem_operator_apply(fem_operator* op, int32_t nrhs, const double* x, double* 
 int fem_operator_time(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps, double* ms);
 /* SpMV schedules: 1, 2, 4, 8 or 16 lanes per row (>= nrhs), or the row-block
  * stream (a workgroup stages its rows' column/value stream in LDS with 16-byte
- * loads), optionally with non-temporal stream loads. */
-enum { FEM_SPMV_STREAM = 32, FEM_SPMV_STREAM_NT = 33 };
+ * loads), optionally with non-temporal stream loads; the pipelined stream
+ * walks the row blocks with a resident grid and loads the next blocks' row
+ * pointers and stream while it gathers the current one. */
+enum { FEM_SPMV_STREAM = 32, FEM_SPMV_STREAM_NT = 33, FEM_SPMV_PIPE = 34, FEM_SPMV_PIPE_NT = 35 };
 /* Schedule autotuner: times every schedule valid for nrhs and keeps the
  * fastest (*schedule receives it). */
 int fem_operator_autotune(fem_operator* op, int32_t nrhs, const void* x, void* y, int32_t reps,

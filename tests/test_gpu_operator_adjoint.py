@@ -55,6 +55,35 @@ def test_operator_vs_oracle_all_schedules(gpu, precision, transpose):
     op.close()
 
 
+@pytest.mark.parametrize("precision", [_lib.FEM_PREC_FP64, _lib.FEM_PREC_FP32_IR])
+def test_operator_pipe_many_blocks_per_workgroup(gpu, precision):
+    """The pipelined stream schedule on a mesh with more row blocks than its
+    resident grid (600 x 600 jittered, ~358K rows: 1,400 blocks of 256 rows
+    at one column, 5,600 of 64 at 16 FP32 columns, 11,200 of 32 at 16 FP64),
+    so every workgroup walks several blocks through both LDS buffers.  Each
+    row's arithmetic is the plain stream schedule's, so the results are
+    bit-identical to it, and they match scipy."""
+    m = meshes.jittered_square(600, seed=4)
+    print("case operator pipe", precision, flush=True)
+    ctx = FEMContext(m, 0)
+    s = oracle.System(m, bc=0)
+    coef = (1.0 / 2e-3, 1.0, -1.0)
+    S = coef[0] * s.scipy("M") + coef[1] * s.scipy("K") + coef[2] * s.scipy("A")
+    op = Operator(ctx, coef, precision)
+    rng = np.random.Generator(np.random.PCG64(12))
+    tol = 1e-14 if precision == _lib.FEM_PREC_FP64 else 2e-6
+    for k in (1, 16):
+        x = rng.standard_normal((s.n, k)) if k > 1 else rng.standard_normal(s.n)
+        op.set_schedule(k, _lib.FEM_SPMV_STREAM)
+        y0 = op.apply(x)
+        for g in (_lib.FEM_SPMV_PIPE, _lib.FEM_SPMV_PIPE_NT):
+            op.set_schedule(k, g)
+            y = op.apply(x)
+            assert np.array_equal(y, y0), (k, g)
+        assert rel(y0, S @ x) <= tol, (k, rel(y0, S @ x))
+    op.close()
+
+
 def test_operator_transpose_dot_product(gpu):
     m, ve, kappa = c5_like(20, 5)
     ctx = FEMContext(m, 0, velocity_e=ve, normalize_velocity=False, kappa_e=kappa)
