@@ -889,8 +889,11 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   const int nact = cv.all ? p->ninst : cv.nact;
   if (nab == 0) return 0;
   SArgs aa = a;
-  aa.act_blk = cv.all ? nullptr : p->d_act_blk;
-  aa.act_inst = cv.all ? nullptr : p->d_act_inst;
+  // one instance: its list is all of its blocks or none, in order, so the
+  // kernels take the identity with the device's counts (k_expand is skipped)
+  const bool lists = !cv.all && p->ninst > 1;
+  aa.act_blk = lists ? p->d_act_blk : nullptr;
+  aa.act_inst = lists ? p->d_act_inst : nullptr;
   aa.act_cnt = cv.all ? nullptr : p->d_cnt;
   void* args[] = {(void*)&aa};
   const bool staged = stream_vec_layout(p->k) && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A ||
@@ -917,7 +920,7 @@ int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, int post =
     FEM_HIP(hipMemcpyAsync(p->h_cnt + 2 * post, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     FEM_HIP(hipEventRecord(p->ev[post], st));
   }
-  if (p->nblocks > 0) {  // queued before the host waits, so it overlaps the read-back
+  if (p->nblocks > 0 && p->ninst > 1) {  // queued before the host waits, so it overlaps the read-back
     hipLaunchKernelGGL(k_expand, dim3(div_up(p->nblocks, kBlock)), dim3(kBlock), 0, st, p->d_blk_inst, p->d_inst_blk,
                        p->d_act_base, p->nblocks, p->d_act_blk);
     FEM_HIP(hipGetLastError());
@@ -926,10 +929,26 @@ int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, int post =
   return 0;
 }
 
-// Wait for the counts queued in `slot`: nab == 0 means no block is left.
-int wait_counts(const StreamPlan* p, int slot, Cover* cv) {
+// One instance: queue the copy of its sweep_done flag into `slot` instead of
+// a compaction (the kernels skip its blocks by the flag itself).
+int post_flag(const StreamPlan* p, int slot, hipStream_t st) {
+  FEM_HIP(hipMemcpyAsync(p->h_cnt + 2 * slot, (const char*)p->d_ss + offsetof(SState, sweep_done), sizeof(int),
+                         hipMemcpyDeviceToHost, st));
+  FEM_HIP(hipEventRecord(p->ev[slot], st));
+  return 0;
+}
+
+// Wait for the counts queued in `slot` (flag: a post_flag read, as counts):
+// nab == 0 means no block is left.
+int wait_counts(const StreamPlan* p, int slot, Cover* cv, bool flag = false) {
   FEM_HIP(hipEventSynchronize(p->ev[slot]));
   cv->all = false;
+  if (flag) {
+    const bool done = p->h_cnt[2 * slot] != 0;
+    cv->nact = done ? 0 : 1;
+    cv->nab = done ? 0 : p->nblocks;
+    return 0;
+  }
   cv->nact = p->h_cnt[2 * slot];
   cv->nab = p->h_cnt[2 * slot + 1];
   return 0;
@@ -1040,10 +1059,14 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
           post = slot;
         }
         int r = 0;
-        if ((check || multi) && (r = compact(p, 0, st, &cv, post))) return r;
+        if (multi) {
+          if ((r = compact(p, 0, st, &cv, post))) return r;
+        } else if (check && (r = post_flag(p, post, st))) {
+          return r;
+        }
         if (pend >= 0) {
           Cover got{};
-          if ((r = wait_counts(p, pend, &got))) return r;
+          if ((r = wait_counts(p, pend, &got, !multi))) return r;
           pend = -1;
           if (got.nab == 0) {  // done after iteration pend_it: this one is empty
             *stop = true;
