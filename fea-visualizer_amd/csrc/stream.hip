@@ -988,6 +988,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   const Cover fixed{false, p->ninst, std::min(ctl->grid, std::max(p->nblocks, 8)), true};
   Cover cv{};
   const bool multi = p->ninst > 1;  // one instance: its list is all of its blocks or none
+  const bool single = !multi;
   std::vector<int> prev_its(std::max(0, max_sweeps), 0);  // iterations each sweep ran in the last step
   if (!budget) {
     ctl->obs_sweeps = 0;
@@ -1018,8 +1019,17 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       FEM_HIP(hipGetLastError());
       // instances that sweep; eager: their counts arrive after the sweep's
       // first launches, which run over the previous sweep's cover meanwhile
+      // (one instance: every launch covers all of its blocks, which skip by
+      // its flags, and the host reads its sweep_done flag)
       int sweep_slot = -1;
-      if (budget) {
+      if (single) {
+        cv = full;
+        if (!budget) {
+          slot ^= 1;
+          sweep_slot = slot;
+          if ((rc = post_flag(p, sweep_slot, st))) return rc;
+        }
+      } else if (budget) {
         if ((rc = compact(p, 0, st, &cv))) return rc;
         cv = fixed;
       } else {
@@ -1032,10 +1042,11 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       bool none = false;
       auto settle = [&]() -> int {
         if (sweep_slot < 0) return 0;
-        const int r = wait_counts(p, sweep_slot, &cv);
+        Cover got{};
+        const int r = wait_counts(p, sweep_slot, &got, single);
         sweep_slot = -1;
-        sweep = cv;
-        none = cv.nab == 0;
+        none = got.nab == 0;
+        if (!single) sweep = cv = got;
         return r;
       };
       // one iteration queued: refresh the lists (several instances), take
@@ -1047,6 +1058,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         ++it_run;
         *stop = false;
         if (budget) {
+          if (single) return 0;
           const int r = compact(p, 0, st, &cv);
           cv = fixed;
           return r;
@@ -1074,8 +1086,10 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
             if (sw < (int)prev_its.size()) prev_its[sw] = pend_it;
             return 0;
           }
-          cv.nact = got.nact;
-          cv.nab = got.nab;
+          if (multi) {
+            cv.nact = got.nact;
+            cv.nab = got.nab;
+          }
         }
         if (post >= 0) {
           pend = post;
@@ -1129,11 +1143,11 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       // then the redo pass for columns that produced non-finite values
       cv = sweep;
       a.pass = 0;
-      if ((rc = compact(p, 1, st, &cv))) return rc;
-      if (budget) cv = fixed;
+      if (!single && (rc = compact(p, 1, st, &cv))) return rc;
+      if (budget && !single) cv = fixed;
       STREAM_PH(PH_RES, cv);
-      if ((rc = compact(p, 2, st, &cv))) return rc;
-      if (budget) cv = fixed;
+      if (!single && (rc = compact(p, 2, st, &cv))) return rc;
+      if (budget && !single) cv = fixed;
       a.pass = 1;
       STREAM_PH(PH_RES, cv);
       a.pass = 0;
