@@ -355,7 +355,8 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
 constexpr int kRedBlock = 1024;
 
 // Sum of the instance's block partials in a fixed order: thread (q, c) sums
-// blocks c, c + chunks, ... of value q; the chunk sums are combined in order.
+// blocks c, c + chunks, ... of value q, kR loads in flight at a time; a fixed
+// pairwise tree combines the chunk sums.
 template <int K, int NS>
 __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* red, double* scratch) {
   constexpr int nv = NS * K;
@@ -364,7 +365,7 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
   const int b0 = inst_blk[i], b1 = inst_blk[i + 1];
   double s = 0.0;
   if (c < chunks) {
-    constexpr int kR = 8;
+    constexpr int kR = 16;
     for (int w0 = b0 + c; w0 < b1; w0 += kR * chunks) {
       double v[kR];
 #pragma unroll
