@@ -1399,10 +1399,9 @@ static int stream_graph_capture(fem_batch* b, int nsteps) {
   // start, a compaction and the two residual passes; per step the right-hand
   // side, the update and the step's end)
   const int kr = b->splan ? b->insts[0].dev.krylov : 0;
-  const long long L = stream_phase_launches(b->splan);  // a reducing phase's launches
-  const long long per_it = L * (kr == 1 ? 3 : kr == 2 ? 2 : 1) + 2;
-  long long per_step = L + 1 + 1;
-  for (int sw = 0; sw < ctl.sweeps; sw++) per_step += 3 + (kr == 0 ? L : 0) + ctl.iters[sw] * per_it + 4 + 2 * L;
+  const long long per_it = 2LL * (kr == 1 ? 3 : kr == 2 ? 2 : 1) + 2;
+  long long per_step = 2 + 1 + 1;
+  for (int sw = 0; sw < ctl.sweeps; sw++) per_step += 3 + (kr == 0 ? 2 : 0) + ctl.iters[sw] * per_it + 8;
   const long long launches = 3 + (long long)nsteps * per_step;
   if (launches > kMaxGraphLaunches) {
     set_error("graph capture (streaming engine): " + std::to_string(nsteps) + " steps would record " +

@@ -62,9 +62,6 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
 // Column counts whose stream phases use the multi-column layout with the
 // staged pattern (one value column, Inst::SDi present).
 bool stream_vec_layout(int k);
-// Launches of one reducing phase: the phase and its reduction, plus the
-// group level of a two-level reduction.
-int stream_phase_launches(const StreamPlan* p);
 int step_max_coresident(int k, int inner_fp32, int kr, int coh, int* blocks);
 int step_graph_node(const StepLaunch& L, int nsteps, hipGraph_t g, hipGraphNode_t dep, hipGraphNode_t* out);
 

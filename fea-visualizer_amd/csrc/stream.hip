@@ -353,17 +353,16 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
 // Threads of a reduction workgroup: the largest instances have thousands of
 // row blocks, and the sum is bound by how many partials are in flight.
 constexpr int kRedBlock = 1024;
-// Row blocks per group of a two-level reduction (k_sgroup).
-constexpr int kGroupBlocks = 256;
 
-// Sum of the partial records [b0, b1) of src ([record][kPartW K]) in a fixed
-// order: thread (q, c) sums records c, c + chunks, ... of value q, kR loads in
-// flight at a time; a fixed pairwise tree combines the chunk sums.
+// Sum of the instance's block partials in a fixed order: thread (q, c) sums
+// blocks c, c + chunks, ... of value q, kR loads in flight at a time; a fixed
+// pairwise tree combines the chunk sums.
 template <int K, int NS>
-__device__ void sum_range(const double* src, int b0, int b1, double* red, double* scratch) {
+__device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* red, double* scratch) {
   constexpr int nv = NS * K;
   constexpr int chunks = kRedBlock / nv;
   const int q = threadIdx.x % nv, c = threadIdx.x / nv;
+  const int b0 = inst_blk[i], b1 = inst_blk[i + 1];
   double s = 0.0;
   if (c < chunks) {
     constexpr int kR = 16;
@@ -372,7 +371,7 @@ __device__ void sum_range(const double* src, int b0, int b1, double* red, double
 #pragma unroll
       for (int u = 0; u < kR; u++) {
         const int w = w0 + u * chunks;
-        v[u] = (w < b1) ? src[(size_t)w * kPartW * K + q] : 0.0;
+        v[u] = (w < b1) ? a.partials[(size_t)w * kPartW * K + q] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kR; u++) s += v[u];
@@ -392,45 +391,22 @@ __device__ void sum_range(const double* src, int b0, int b1, double* red, double
   __syncthreads();
 }
 
-// Instances whose phase launch did nothing: their partials are not reduced.
-template <int PH>
-__device__ inline bool reduce_skip(const SArgs& a, const SState& S) {
-  if constexpr (PH == PH_RES) return a.pass ? !S.redo : S.solve_done;
-  else if constexpr (PH != PH_RHS) return S.sweep_done != 0;
-  else return false;
-}
-
-// Instances with many row blocks (two-level plans): one block per group of
-// kGroupBlocks row blocks sums their partials into the group's record, so a
-// large instance is summed by many CUs instead of one.
+// One block per instance: reduce the phase's partials and advance the scalar
+// recurrences exactly as the persistent drivers do after their barriers.
 template <int K, int PH>
-__global__ void __launch_bounds__(kRedBlock) k_sgroup(SArgs a, const int* grp_inst, const int* grp_b0,
-                                                      const int* grp_b1, int ngrp, double* gpart) {
-  __shared__ double scratch[kRedBlock];
-  __shared__ double red[kPartW * kMaxK];
-  const int g = blockIdx.x;
-  if (g >= ngrp) return;
-  if (reduce_skip<PH>(a, a.ss[grp_inst[g]])) return;
-  constexpr int NS = nsums<K, PH>();
-  sum_range<K, NS>(a.partials, grp_b0[g], grp_b1[g], red, scratch);
-  if (threadIdx.x < NS * K) gpart[(size_t)g * kPartW * K + threadIdx.x] = red[threadIdx.x];
-}
-
-// One block per instance: reduce the phase's partial records src[ranges[i],
-// ranges[i + 1]) (the row blocks', or the groups' of a two-level plan) and
-// advance the scalar recurrences exactly as the persistent drivers do after
-// their barriers.
-template <int K, int PH>
-__global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const double* src, const int* ranges, int ninst,
-                                                       int fp32) {
+__global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_blk, int ninst, int fp32) {
   __shared__ double scratch[kRedBlock];
   __shared__ double red[kPartW * kMaxK];
   if ((int)blockIdx.x >= ninst || (a.act_cnt && (int)blockIdx.x >= a.act_cnt[0])) return;
   const int i = a.act_inst ? a.act_inst[blockIdx.x] : blockIdx.x;
   const Inst& I = a.insts[i];
   SState& S = a.ss[i];
-  if (reduce_skip<PH>(a, S)) return;
-  sum_range<K, nsums<K, PH>()>(src, ranges[i], ranges[i + 1], red, scratch);
+  if constexpr (PH == PH_RES) {
+    if (a.pass ? !S.redo : S.solve_done) return;
+  } else if constexpr (PH != PH_RHS) {
+    if (S.sweep_done) return;
+  }
+  sum_blocks<K, nsums<K, PH>()>(a, inst_blk, i, red, scratch);
   if (threadIdx.x != 0) return;
   if constexpr (PH == PH_RHS) {
     int any = 0;
@@ -727,22 +703,20 @@ const void* phase_fn(int ph, bool walk) {
   }
 }
 
-template <int K, bool G>
+template <int K>
 const void* reduce_fn(int ph) {
-#define FEM_RED(P) (G ? (const void*)k_sgroup<K, P> : (const void*)k_sreduce<K, P>)
   switch (ph) {
-    case PH_RHS: return FEM_RED(PH_RHS);
-    case PH_RES: return FEM_RED(PH_RES);
-    case PH_PCG_INIT: return FEM_RED(PH_PCG_INIT);
-    case PH_PCG_ITER: return FEM_RED(PH_PCG_ITER);
-    case PH_CG_A: return FEM_RED(PH_CG_A);
-    case PH_CG_B: return FEM_RED(PH_CG_B);
-    case PH_BI_P: return FEM_RED(PH_BI_P);
-    case PH_BI_S: return FEM_RED(PH_BI_S);
-    case PH_BI_X: return FEM_RED(PH_BI_X);
+    case PH_RHS: return (const void*)k_sreduce<K, PH_RHS>;
+    case PH_RES: return (const void*)k_sreduce<K, PH_RES>;
+    case PH_PCG_INIT: return (const void*)k_sreduce<K, PH_PCG_INIT>;
+    case PH_PCG_ITER: return (const void*)k_sreduce<K, PH_PCG_ITER>;
+    case PH_CG_A: return (const void*)k_sreduce<K, PH_CG_A>;
+    case PH_CG_B: return (const void*)k_sreduce<K, PH_CG_B>;
+    case PH_BI_P: return (const void*)k_sreduce<K, PH_BI_P>;
+    case PH_BI_S: return (const void*)k_sreduce<K, PH_BI_S>;
+    case PH_BI_X: return (const void*)k_sreduce<K, PH_BI_X>;
     default: return nullptr;
   }
-#undef FEM_RED
 }
 
 const void* pick_phase(int k, int fp32, int ph, bool walk) {
@@ -755,13 +729,12 @@ const void* pick_phase(int k, int fp32, int ph, bool walk) {
   }
 }
 
-// group: the first level of a two-level reduction (k_sgroup)
-const void* pick_reduce(int k, int ph, bool group) {
+const void* pick_reduce(int k, int ph) {
   switch (k) {
-    case 1: return group ? reduce_fn<1, true>(ph) : reduce_fn<1, false>(ph);
-    case 2: return group ? reduce_fn<2, true>(ph) : reduce_fn<2, false>(ph);
-    case 4: return group ? reduce_fn<4, true>(ph) : reduce_fn<4, false>(ph);
-    case 16: return group ? reduce_fn<16, true>(ph) : reduce_fn<16, false>(ph);
+    case 1: return reduce_fn<1>(ph);
+    case 2: return reduce_fn<2>(ph);
+    case 4: return reduce_fn<4>(ph);
+    case 16: return reduce_fn<16>(ph);
     default: return nullptr;
   }
 }
@@ -773,7 +746,6 @@ const void* pick_reduce(int k, int ph, bool group) {
 // ---------------------------------------------------------------------------
 
 bool stream_vec_layout(int k) { return k == 1 || k >= 4; }
-
 
 // Pinned slots for the host's reads of the list counts: a read is queued in
 // one slot while the previous one is still being waited for.
@@ -793,24 +765,14 @@ struct StreamPlan {
   int* d_act_base = nullptr;  // [ninst]
   int* d_act_blk = nullptr;   // [nblocks]
   int* d_cnt = nullptr;       // {active instances, active blocks}
-  // two-level reductions (several instances, one of more than kGroupBlocks
-  // row blocks): groups of up to kGroupBlocks consecutive row blocks of one
-  // instance, and the group records [ngrp][kPartW k]
-  int ngrp = 0;
-  int* d_grp_inst = nullptr;
-  int* d_grp_b0 = nullptr;
-  int* d_grp_b1 = nullptr;
-  int* d_inst_grp = nullptr;  // [ninst + 1]
-  double* d_gpart = nullptr;
   int* h_cnt = nullptr;       // pinned copies: kCountSlots x {instances, blocks}
   hipEvent_t ev[kCountSlots] = {};  // marks each slot's copy
 };
 
 void stream_plan_free(StreamPlan* p) {
   if (!p) return;
-  void* ptrs[] = {p->d_ss,       p->d_blk_inst, p->d_blk_row,  p->d_blk_end,  p->d_inst_blk,
-                  p->d_partials, p->d_act_inst, p->d_act_base, p->d_act_blk,   p->d_cnt,
-                  p->d_grp_inst, p->d_grp_b0,   p->d_grp_b1,   p->d_inst_grp, p->d_gpart};
+  void* ptrs[] = {p->d_ss,       p->d_blk_inst, p->d_blk_row,  p->d_blk_end, p->d_inst_blk,
+                  p->d_partials, p->d_act_inst, p->d_act_base, p->d_act_blk,  p->d_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->h_cnt) (void)hipHostFree(p->h_cnt);
@@ -821,8 +783,6 @@ void stream_plan_free(StreamPlan* p) {
 
 // Row-block table over all instances: up to `rows` rows per block, cut
 // shorter where the block's pattern would exceed kStageCap nonzeros.
-int stream_phase_launches(const StreamPlan* p) { return p && p->ngrp > 0 ? 3 : 2; }
-
 int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, int k, int fp32, int kr, int rows,
                       hipStream_t st, StreamPlan** out) {
   (void)d_insts;
@@ -860,23 +820,6 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   }
   ib[p->ninst] = (int)bi.size();
   p->nblocks = (int)bi.size();
-  // two-level reductions when one instance of several has more than
-  // kGroupBlocks row blocks (one instance: its reduction block has the GPU)
-  std::vector<int> gi, g0, g1, ig(p->ninst + 1, 0);
-  int max_nb = 0;
-  for (int i = 0; i < p->ninst; i++) max_nb = std::max(max_nb, ib[i + 1] - ib[i]);
-  if (p->ninst > 1 && max_nb > kGroupBlocks) {
-    for (int i = 0; i < p->ninst; i++) {
-      ig[i] = (int)gi.size();
-      for (int b = ib[i]; b < ib[i + 1]; b += kGroupBlocks) {
-        gi.push_back(i);
-        g0.push_back(b);
-        g1.push_back(std::min(ib[i + 1], b + kGroupBlocks));
-      }
-    }
-    ig[p->ninst] = (int)gi.size();
-    p->ngrp = (int)gi.size();
-  }
   const size_t tb = fp32 ? 4 : 8;
   p->lds = 4 * (size_t)((p->stage_rows + 4) & ~3) + 4 * (size_t)((p->stage_nnz + 3) & ~3) + tb * p->stage_nnz;
   if (stream_vec_layout(k) && p->lds > kMaxLds) {
@@ -895,13 +838,6 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->d_act_base = dalloc<int>(p->ninst, &rc);
   p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_cnt = dalloc<int>(2, &rc);
-  if (p->ngrp > 0) {
-    p->d_grp_inst = dalloc<int>(p->ngrp, &rc);
-    p->d_grp_b0 = dalloc<int>(p->ngrp, &rc);
-    p->d_grp_b1 = dalloc<int>(p->ngrp, &rc);
-    p->d_inst_grp = dalloc<int>(p->ninst + 1, &rc);
-    p->d_gpart = dalloc<double>((size_t)p->ngrp * kPartW * k, &rc);
-  }
   if (!rc) {
     hipError_t e = hipHostMalloc((void**)&p->h_cnt, 2 * kCountSlots * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
@@ -920,12 +856,6 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
     FEM_HIP(hipMemcpyAsync(p->d_blk_end, be.data(), sizeof(int) * be.size(), hipMemcpyHostToDevice, st));
   }
   FEM_HIP(hipMemcpyAsync(p->d_inst_blk, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice, st));
-  if (p->ngrp > 0) {
-    FEM_HIP(hipMemcpyAsync(p->d_grp_inst, gi.data(), sizeof(int) * gi.size(), hipMemcpyHostToDevice, st));
-    FEM_HIP(hipMemcpyAsync(p->d_grp_b0, g0.data(), sizeof(int) * g0.size(), hipMemcpyHostToDevice, st));
-    FEM_HIP(hipMemcpyAsync(p->d_grp_b1, g1.data(), sizeof(int) * g1.size(), hipMemcpyHostToDevice, st));
-    FEM_HIP(hipMemcpyAsync(p->d_inst_grp, ig.data(), sizeof(int) * ig.size(), hipMemcpyHostToDevice, st));
-  }
   FEM_HIP(hipStreamSynchronize(st));
   *out = p;
   return 0;
@@ -971,23 +901,10 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
                                                   ph == PH_BI_P || ph == PH_BI_S);
   FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, staged ? p->lds : 0, st));
   if (ph == PH_UPD) return 0;
-  const double* src = p->d_partials;
-  const int* ranges = p->d_inst_blk;
-  if (p->ngrp > 0) {  // first level: every group of every instance (skipped ones exit)
-    const void* gf = pick_reduce(p->k, ph, true);
-    const int* gi = p->d_grp_inst;
-    const int* g0 = p->d_grp_b0;
-    const int* g1 = p->d_grp_b1;
-    int ngrp = p->ngrp;
-    double* gpart = p->d_gpart;
-    void* gargs[] = {(void*)&aa, (void*)&gi, (void*)&g0, (void*)&g1, (void*)&ngrp, (void*)&gpart};
-    FEM_HIP(hipLaunchKernel(gf, dim3(ngrp), dim3(kRedBlock), gargs, 0, st));
-    src = p->d_gpart;
-    ranges = p->d_inst_grp;
-  }
-  const void* rf = pick_reduce(p->k, ph, false);
+  const void* rf = pick_reduce(p->k, ph);
+  const int* ib = p->d_inst_blk;
   int fp32 = p->fp32;
-  void* rargs[] = {(void*)&aa, (void*)&src, (void*)&ranges, (void*)&nact, (void*)&fp32};
+  void* rargs[] = {(void*)&aa, (void*)&ib, (void*)&nact, (void*)&fp32};
   FEM_HIP(hipLaunchKernel(rf, dim3(nact), dim3(kRedBlock), rargs, 0, st));
   return 0;
 }
