@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE (calibrated against a plain copy kernel) for the
 # exact command given; the summaries with their commands are in profiles/.
 PROFILED_TRAFFIC = {
-    "grayscott": {"steps": 20, "warmup": 5, "bytes_per_launch": 1.5399e9,
+    "grayscott": {"steps": 20, "warmup": 5, "predictor": "cubic", "bytes_per_launch": 1.5399e9,
                   "source": "profiles/k_step_grayscott_icosphere6.md"},
     "spmv": {"bytes_per_launch": 1.2087e9, "source": "profiles/k_spmv_stream_jittered4096.md"},
 }
@@ -726,7 +726,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="cubic",
+    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="quartic",
                     help="grayscott, batched: initial guess of each solve")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")
@@ -866,7 +866,8 @@ def grayscott_line(args):
     bytes_alg = step_bytes(n, nnz, 2, 2, st.iterations, st.sweeps, K, fused=True)
     ach = bytes_alg / (kernel_ms * 1e-3) / 1e9
     prof = PROFILED_TRAFFIC["grayscott"]
-    same_cmd = (args.steps, args.warmup, args.level, args.tol) == (prof["steps"], prof["warmup"], 6, 1e-10)
+    same_cmd = ((args.steps, args.warmup, args.level, args.tol, args.predictor, args.inner_tol, args.team_size)
+                == (prof["steps"], prof["warmup"], 6, 1e-10, prof["predictor"], None, 0))
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS,
                        "traffic": prof["bytes_per_launch"] if same_cmd else None,

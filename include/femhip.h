@@ -126,12 +126,13 @@ typedef struct fem_solver_opts {
   int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or
                             autotuner), _COHERENT or _FENCED */
   int32_t predictor;     /* initial guess with warm_start, from the previous solutions:
-                            FEM_PREDICTOR_CUBIC (the fem_opts_default choice: 4 x_n -
-                            6 x_{n-1} + 4 x_{n-2} - x_{n-3}; one more state-sized history
-                            vector), FEM_PREDICTOR_QUADRATIC (3 x_n - 3 x_{n-1} + x_{n-2}),
-                            FEM_PREDICTOR_QUARTIC (5 x_n - 10 x_{n-1} + 10 x_{n-2} -
-                            5 x_{n-3} + x_{n-4}), FEM_PREDICTOR_LINEAR (0: 2 x_n -
-                            x_{n-1}) or FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
+                            FEM_PREDICTOR_QUARTIC (the fem_opts_default choice: 5 x_n -
+                            10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4}; two more
+                            state-sized history vectors), FEM_PREDICTOR_CUBIC (4 x_n -
+                            6 x_{n-1} + 4 x_{n-2} - x_{n-3}; one more),
+                            FEM_PREDICTOR_QUADRATIC (3 x_n - 3 x_{n-1} + x_{n-2}),
+                            FEM_PREDICTOR_LINEAR (0: 2 x_n - x_{n-1}) or
+                            FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
                             Gray-Scott take any order; the wave equation is linear at most;
                             Newmark extrapolates its acceleration, quadratic at most. */
 } fem_solver_opts;
