@@ -707,8 +707,8 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
       if (rc) return rc;
       b->plan_ready = false;
     }
-    // the cubic predictor's third history vector, for the equations that use it
-    // history vectors beyond `a` and `v` the predictor needs (h3, h4)
+    // history vectors beyond `a` and `v` the predictor needs (h3, h4), for the
+    // equations that use them
     const int pr = b->opts.predictor;
     int nh = 0;
     if (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)
