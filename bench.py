@@ -70,7 +70,8 @@ def gs_initial(mesh, n, seed_centres=0, seed_noise=1, npatch=50, radius=0.05):
     return u, v
 
 
-PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3, "quartic": 4}  # FEM_PREDICTOR_*
+PREDICTORS = {"linear": 0, "previous": 1, "quadratic": 2, "cubic": 3, "quartic": 4, "quintic": 5,
+              "sextic": 6}  # FEM_PREDICTOR_*
 
 
 def step_bytes(n, nnz, k, kv, iters, sweeps, steps, ti=4, bicgstab=False, fused=False, stream=False):
@@ -726,7 +727,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--team-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="quartic",
+    ap.add_argument("--predictor", choices=sorted(PREDICTORS), default="sextic",
                     help="grayscott, batched: initial guess of each solve")
     ap.add_argument("--engine", choices=["auto", "stream"], default="auto",
                     help="batched, wave: 'stream' runs the streaming engine without autotuning")

@@ -64,7 +64,7 @@ void fem_opts_default(fem_solver_opts* o) {
   o->warm_start = 1;
   o->tol = 1e-10;
   o->inner_tol = 1e-4;
-  o->predictor = FEM_PREDICTOR_QUARTIC;
+  o->predictor = FEM_PREDICTOR_SEXTIC;
 }
 
 // ------------------------------------------------------------------ context

@@ -99,7 +99,7 @@ __global__ void k_get_cols(const double* src, int k, int col0, double* dst, int 
 void free_inst(InstHost& h) {
   void* ptrs[] = {h.d_S64, h.d_Si,    h.d_dinv64, h.d_dinvi, h.d_SDi, h.d_u, h.d_v,   h.d_a,
                   h.d_h3,  h.d_h4,    h.d_xnext,  h.d_hpre,  h.d_hvec, h.d_x, h.d_b, h.d_vec,
-                  h.d_surf};
+                  h.d_surf, h.d_h5, h.d_h6};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h.surf_done) (void)hipEventSynchronize(h.surf_done);
@@ -269,6 +269,8 @@ void fill_dev(fem_batch* b, InstHost& h) {
   I.a = h.d_a;
   I.h3 = h.d_h3;
   I.h4 = h.d_h4;
+  I.h5 = h.d_h5;
+  I.h6 = h.d_h6;
   I.xnext = h.d_xnext;
   I.hpre = h.d_hpre;
   I.hvec = h.d_hvec;
@@ -292,6 +294,10 @@ void fill_dev(fem_batch* b, InstHost& h) {
     else if (o.predictor == FEM_PREDICTOR_QUADRATIC) I.extrap = first_order ? 2 : 1;
     else if (o.predictor == FEM_PREDICTOR_CUBIC) I.extrap = !first_order ? 1 : h.d_h3 ? 3 : 2;
     else if (o.predictor == FEM_PREDICTOR_QUARTIC) I.extrap = !first_order ? 1 : h.d_h4 ? 4 : h.d_h3 ? 3 : 2;
+    else if (o.predictor == FEM_PREDICTOR_QUINTIC || o.predictor == FEM_PREDICTOR_SEXTIC) {
+      const int want = o.predictor == FEM_PREDICTOR_SEXTIC ? 6 : 5;  // the persistent plan uses 4 at most
+      I.extrap = !first_order ? 1 : (want >= 6 && h.d_h6) ? 6 : h.d_h5 ? 5 : h.d_h4 ? 4 : h.d_h3 ? 3 : 2;
+    }
   } else if (o.warm_start && h.eq_override < 0 && h.eq == FEM_WAVE_NEWMARK && o.predictor != FEM_PREDICTOR_PREVIOUS) {
     // Newmark extrapolates the acceleration (history in h3, h4): linear or quadratic
     I.extrap = (o.predictor != FEM_PREDICTOR_LINEAR && h.d_h4) ? 2 : h.d_h3 ? 1 : 0;
@@ -377,6 +383,15 @@ namespace {
 // grid.  With many, instances heavier than the per-workgroup share get teams
 // proportional to their work; the rest are packed longest-first onto
 // single-workgroup teams (each walks its instances in turn).
+
+// The persistent kernel's copy of a descriptor: its predictor is quartic at
+// most (the history levels beyond h4 belong to the streaming engine).
+static Inst persistent_desc(const Inst& d) {
+  Inst I = d;
+  I.extrap = std::min(I.extrap, kPersistentExtrap);
+  return I;
+}
+
 int build_plan(fem_batch* b) {
   free_plan(b);
   const int ninst = (int)b->insts.size();
@@ -531,7 +546,7 @@ int build_plan(fem_batch* b) {
     poff += sizes[t];
   }
   std::vector<Inst> hi(ninst);
-  for (int i = 0; i < ninst; i++) hi[i] = b->insts[i].dev;
+  for (int i = 0; i < ninst; i++) hi[i] = persistent_desc(b->insts[i].dev);
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, b->stream));
   b->uploaded = hi;
   FEM_HIP(hipMemcpyAsync(b->d_teams, ht.data(), sizeof(Team) * nteams, hipMemcpyHostToDevice, b->stream));
@@ -712,11 +727,15 @@ int batch_set_equation(fem_batch* b, int inst, int eq, const void* params, size_
     const int pr = b->opts.predictor;
     int nh = 0;
     if (eq == FEM_HEAT || eq == FEM_ADVECTION_DIFFUSION || eq == FEM_REACTION_DIFFUSION)
-      nh = pr == FEM_PREDICTOR_CUBIC ? 1 : pr == FEM_PREDICTOR_QUARTIC ? 2 : 0;
+      nh = pr == FEM_PREDICTOR_CUBIC     ? 1
+           : pr == FEM_PREDICTOR_QUARTIC ? 2
+           : pr == FEM_PREDICTOR_QUINTIC ? 3
+           : pr == FEM_PREDICTOR_SEXTIC  ? 4
+                                         : 0;
     else if (eq == FEM_WAVE_NEWMARK)
       nh = pr == FEM_PREDICTOR_PREVIOUS ? 0 : pr == FEM_PREDICTOR_LINEAR ? 1 : 2;
     for (int q = 0; q < nh; q++) {
-      double** hp = q == 0 ? &h.d_h3 : &h.d_h4;
+      double** hp = q == 0 ? &h.d_h3 : q == 1 ? &h.d_h4 : q == 2 ? &h.d_h5 : &h.d_h6;
       if (*hp) continue;
       int rc = 0;
       *hp = dalloc<double>((size_t)h.ctx->n * k, &rc);
@@ -1041,12 +1060,11 @@ int batch_prepare(fem_batch* b, bool need_plan) {
   if (!b->plan_ready) return build_plan(b);
   // descriptors may have changed since the plan was built (trajectory
   // records, equation override, predictor history): upload them when they did
-  bool same = b->uploaded.size() == b->insts.size();
-  for (size_t i = 0; same && i < b->insts.size(); i++)
-    same = std::memcmp(&b->uploaded[i], &b->insts[i].dev, sizeof(Inst)) == 0;
-  if (same) return 0;
   std::vector<Inst> hi(b->insts.size());
-  for (size_t i = 0; i < hi.size(); i++) hi[i] = b->insts[i].dev;
+  for (size_t i = 0; i < hi.size(); i++) hi[i] = persistent_desc(b->insts[i].dev);
+  bool same = b->uploaded.size() == hi.size();
+  for (size_t i = 0; same && i < hi.size(); i++) same = std::memcmp(&b->uploaded[i], &hi[i], sizeof(Inst)) == 0;
+  if (same) return 0;
   FEM_HIP(hipMemcpyAsync(b->d_insts, hi.data(), sizeof(Inst) * hi.size(), hipMemcpyHostToDevice, b->stream));
   FEM_HIP(hipStreamSynchronize(b->stream));
   b->uploaded = hi;
@@ -1342,8 +1360,8 @@ static void stream_turn_host(fem_batch* b, int turned) {
     Inst& I = h.dev;
     if (!first_order_eq(I.eq) || I.extrap < 2) continue;
     for (int t = 0; t < turned % I.extrap; t++) {
-      turn_history(&h.d_a, &h.d_v, &h.d_h3, &h.d_h4, I.extrap);
-      turn_history(&I.a, &I.v, &I.h3, &I.h4, I.extrap);
+      turn_history(&h.d_a, &h.d_v, &h.d_h3, &h.d_h4, &h.d_h5, &h.d_h6, I.extrap);
+      turn_history(&I.a, &I.v, &I.h3, &I.h4, &I.h5, &I.h6, I.extrap);
     }
   }
 }
@@ -1499,7 +1517,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
   size_t want = 0;
   for (auto& h : b->insts) {
     const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4}) {
+    for (double* src : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4, h.d_h5, h.d_h6}) {
       if (!src) continue;
       want++;
       if (rc) continue;
@@ -1520,7 +1538,7 @@ int batch_autotune(fem_batch* b, int probe, int* chosen, double* best_ms) {
     size_t q = 0;
     for (auto& h : b->insts) {
       const size_t vb = (size_t)h.ctx->n * h.k * sizeof(double);
-      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4})
+      for (double* dst : {h.d_u, h.d_v, h.d_a, h.d_x, h.d_h3, h.d_h4, h.d_h5, h.d_h6})
         if (dst) FEM_HIP(hipMemcpyAsync(dst, snap[q++], vb, hipMemcpyDeviceToDevice, b->stream));
     }
     FEM_HIP(hipMemcpyAsync(b->d_info, info0, b->info_bytes, hipMemcpyDeviceToDevice, b->stream));

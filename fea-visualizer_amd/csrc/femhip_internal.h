@@ -33,11 +33,16 @@ enum Eq : int {
 // history (Inst::extrap levels) lives in a, v, h3, h4, newest first.
 __host__ __device__ inline bool first_order_eq(int eq) { return eq == EQ_HEAT || eq == EQ_ADVDIFF || eq == EQ_GS; }
 
+// Predictor orders: up to quartic on the persistent kernel (history in a, v,
+// h3, h4), up to sextic on the streaming engine (also h5, h6).
+constexpr int kMaxExtrap = 6, kPersistentExtrap = 4;
+
 // One turn of that history by pointer: the oldest of the e slots becomes the
 // newest (the streaming engine's update has written u_n into it).
-__host__ __device__ inline void turn_history(double** a, double** v, double** h3, double** h4, int e) {
-  double** s[4] = {a, v, h3, h4};
-  if (e < 2 || e > 4) return;
+__host__ __device__ inline void turn_history(double** a, double** v, double** h3, double** h4, double** h5,
+                                             double** h6, int e) {
+  double** s[kMaxExtrap] = {a, v, h3, h4, h5, h6};
+  if (e < 2 || e > kMaxExtrap) return;
   double* t = *s[e - 1];
   for (int i = e - 1; i > 0; i--) *s[i] = *s[i - 1];
   *s[0] = t;
@@ -111,7 +116,7 @@ struct Inst {
   int hist;              // history levels used at launch start (0..extrap)
   int hist_raw;          // history levels held at launch start, before the cap by extrap
   int* hist_dev;         // optional: the held levels in device memory; the stepping kernels
-                         //   read it at launch start and store min(4, levels + nsteps) at the end,
+                         //   read it at launch start and store min(order cap, levels + nsteps) at the end,
                          //   and a state write that changes the state stores 0
   double* h3;            // [n][k] x_{n-3} (cubic and quartic predictors)
   double* h4;            // [n][k] x_{n-4} (quartic predictor: 5 x_n - 10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4})
@@ -120,6 +125,8 @@ struct Inst {
   double* hvec;          // [n][k] history part of the next initial guess, formed with the update:
                          //   phase_res gathers it instead of the history levels; Newmark on the
                          //   streaming engine: the next u + dt v + (1/2 - beta) dt^2 a
+  double* h5;            // [n][k] x_{n-5}, x_{n-6} (quintic and sextic predictors, streaming engine)
+  double* h6;
 };
 
 // A team: a contiguous range of workgroups that advances a list of instances

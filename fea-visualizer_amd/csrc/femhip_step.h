@@ -81,6 +81,8 @@ struct InstHost {
   int traj_every = 0;
   double* d_h3 = nullptr;    // cubic / quartic predictor history
   double* d_h4 = nullptr;
+  double* d_h5 = nullptr;    // quintic / sextic history (streaming engine)
+  double* d_h6 = nullptr;
   double* d_xnext = nullptr; // Inst::xnext, Inst::hpre
   double* d_hpre = nullptr;
   double* d_hvec = nullptr;  // Inst::hvec

@@ -376,7 +376,8 @@ __device__ void phase_rhs(const Inst& I, int row_lo, int row_hi, TI* r, int step
   // predictor order this step: 2 x_n - x_{n-1} once the previous solution is
   // known, 3 x_n - 3 x_{n-1} + x_{n-2} once two are, 4 x_n - 6 x_{n-1} +
   // 4 x_{n-2} - x_{n-3} once three are
-  const int lv = warm ? min(I.extrap, I.hist + step) : 0;
+  // (the guess gathered from the history levels here: quartic at most)
+  const int lv = warm ? min(min(I.extrap, kPersistentExtrap), I.hist + step) : 0;
   const bool ex = lv > 0;
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
@@ -483,6 +484,20 @@ __device__ inline void pred_weights(int lv, double& w0, double& w1, double& w2, 
   w2 = lv == 4 ? 10.0 : lv == 3 ? 4.0 : lv == 2 ? 1.0 : 0.0;
   w3 = lv == 4 ? -5.0 : lv == 3 ? -1.0 : 0.0;
   w4 = lv == 4 ? 1.0 : 0.0;
+}
+
+// The same weights for any order up to kMaxExtrap: w[j] = (-1)^j C(lv + 1, j + 1)
+// on x_{n-j} (lv = 0: x_n), zero beyond the order.
+__device__ inline void pred_weights_n(int lv, double (&w)[kMaxExtrap + 1]) {
+  double c = 1.0;
+  for (int j = 0; j <= kMaxExtrap; j++) {
+    if (j <= lv) {
+      c = c * (double)(lv + 1 - j) / (double)(j + 1);
+      w[j] = (j & 1) ? -c : c;
+    } else {
+      w[j] = 0.0;
+    }
+  }
 }
 
 // LDS row cache of a workgroup's rows (filled by fill_cache below).

@@ -150,9 +150,9 @@ __device__ void stream_update_first(const Inst& I, int row_lo, int row_hi, const
   const int jr = threadIdx.x % K;
   const TI GA* xi = gp((const TI*)I.xi);
   const int e = I.extrap;
-  double GA* oldest = gp(e == 1 ? I.a : e == 2 ? I.v : e == 3 ? I.h3 : I.h4);  // e >= 1
-  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0;
-  if (guess) pred_weights(lvn, w0, w1, w2, w3, w4);
+  double GA* oldest = gp(e == 1 ? I.a : e == 2 ? I.v : e == 3 ? I.h3 : e == 4 ? I.h4 : e == 5 ? I.h5 : I.h6);
+  double w[kMaxExtrap + 1] = {};
+  if (guess) pred_weights_n(lvn, w);
   int f = 0;
   double* traj = nullptr;  // trajectory record of this step, if any
   if (I.traj && (step + 1) % I.traj_every == 0)
@@ -164,8 +164,10 @@ __device__ void stream_update_first(const Inst& I, int row_lo, int row_hi, const
     const double ao = e >= 2 ? gp(I.a)[o] : 0.0;
     const double vo = e >= 3 ? gp(I.v)[o] : 0.0;
     const double ho = e >= 4 ? gp(I.h3)[o] : 0.0;
+    const double h4o = e >= 5 ? gp(I.h4)[o] : 0.0;
+    const double h5o = e >= 6 ? gp(I.h5)[o] : 0.0;
     if (sh.bb[jr] == 0.0) xf = 0.0;  // Eigen: rhs == 0 -> x = 0
-    if (guess) gp(I.x)[o] = w0 * xf + w1 * uo + w2 * ao + w3 * vo + w4 * ho;
+    if (guess) gp(I.x)[o] = w[0] * xf + w[1] * uo + w[2] * ao + w[3] * vo + w[4] * ho + w[5] * h4o + w[6] * h5o;
     if (e >= 1) oldest[o] = uo;
     gp(I.u)[o] = xf;
     if (traj) gp(traj)[o] = xf;
@@ -576,7 +578,7 @@ __global__ void k_step_end(SState* ss, Inst* insts, int ninst, int k) {
   if (i >= ninst) return;
   SState& S = ss[i];
   Inst& I = insts[i];
-  if (first_order_eq(I.eq)) turn_history(&I.a, &I.v, &I.h3, &I.h4, I.extrap);
+  if (first_order_eq(I.eq)) turn_history(&I.a, &I.v, &I.h3, &I.h4, &I.h5, &I.h6, I.extrap);
   int notconv = 0;
   for (int j = 0; j < k; j++) {
     notconv |= S.active[j];
@@ -601,7 +603,7 @@ __global__ void k_hist_load(Inst* insts, int ninst) {
 __global__ void k_hist_store(const Inst* insts, int ninst, int nsteps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst || !insts[i].hist_dev) return;
-  *insts[i].hist_dev = min(4, insts[i].hist_raw + nsteps);
+  *insts[i].hist_dev = min(kMaxExtrap, insts[i].hist_raw + nsteps);
 }
 
 __global__ void k_sinit(SState* ss, const Inst* insts, int ninst) {

@@ -155,7 +155,9 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
   const TI GA* dinv = gp((const TI*)I.dinvi);
   const double inv_dt = 1.0 / I.dt;
   const bool warm = I.warm != 0;
-  const int lv = warm ? min(I.extrap, I.hist + step) : 0;  // predictor order this step
+  // predictor order this step where the guess is gathered from the history
+  // levels (the first step of a launch): quartic at most
+  const int lv = warm ? min(min(I.extrap, kPersistentExtrap), I.hist + step) : 0;
   const bool ex = lv > 0;
   // after the first step of a launch the update wrote this step's guess into
   // x (stream_update_first, stream_update_newmark, stream.hip) and, for

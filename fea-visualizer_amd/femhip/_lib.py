@@ -59,6 +59,8 @@ FEM_PREDICTOR_PREVIOUS = 1
 FEM_PREDICTOR_QUADRATIC = 2
 FEM_PREDICTOR_CUBIC = 3
 FEM_PREDICTOR_QUARTIC = 4
+FEM_PREDICTOR_QUINTIC = 5
+FEM_PREDICTOR_SEXTIC = 6
 FEM_MESH_OPEN = 0
 FEM_MESH_CLOSED = 1
 FEM_MESH_MIRRORED = 2

@@ -143,7 +143,7 @@ class SolverOptions:
     tol: float = 1e-10
     inner_tol: float = 1e-4
     exchange: int = L.FEM_EXCHANGE_AUTO
-    predictor: int = L.FEM_PREDICTOR_QUARTIC
+    predictor: int = L.FEM_PREDICTOR_SEXTIC
 
     def to_c(self):
         o = L.fem_solver_opts()

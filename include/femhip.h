@@ -126,13 +126,18 @@ typedef struct fem_solver_opts {
   int32_t exchange;      /* multi-workgroup teams: FEM_EXCHANGE_AUTO (row-cache rule or
                             autotuner), _COHERENT or _FENCED */
   int32_t predictor;     /* initial guess with warm_start, from the previous solutions:
-                            FEM_PREDICTOR_QUARTIC (the fem_opts_default choice: 5 x_n -
-                            10 x_{n-1} + 10 x_{n-2} - 5 x_{n-3} + x_{n-4}; two more
-                            state-sized history vectors), FEM_PREDICTOR_CUBIC (4 x_n -
+                            FEM_PREDICTOR_SEXTIC (the fem_opts_default choice; see below),
+                            FEM_PREDICTOR_QUARTIC (5 x_n - 10 x_{n-1} + 10 x_{n-2} -
+                            5 x_{n-3} + x_{n-4}; two more state-sized history vectors),
+                            FEM_PREDICTOR_CUBIC (4 x_n -
                             6 x_{n-1} + 4 x_{n-2} - x_{n-3}; one more),
                             FEM_PREDICTOR_QUADRATIC (3 x_n - 3 x_{n-1} + x_{n-2}),
-                            FEM_PREDICTOR_LINEAR (0: 2 x_n - x_{n-1}) or
-                            FEM_PREDICTOR_PREVIOUS (x_n).  Heat, advection-diffusion and
+                            FEM_PREDICTOR_LINEAR (0: 2 x_n - x_{n-1}),
+                            FEM_PREDICTOR_PREVIOUS (x_n), or on the streaming engine
+                            FEM_PREDICTOR_QUINTIC / _SEXTIC (orders 5 and 6, weights
+                            (-1)^j C(order + 1, j + 1) on x_{n-j}; three and four more
+                            vectors; the persistent engine and the first step of each
+                            streaming launch use quartic at most).  Heat, advection-diffusion and
                             Gray-Scott take any order; the wave equation is linear at most;
                             Newmark extrapolates its acceleration, quadratic at most. */
 } fem_solver_opts;
@@ -142,7 +147,9 @@ enum {
   FEM_PREDICTOR_PREVIOUS = 1,
   FEM_PREDICTOR_QUADRATIC = 2,
   FEM_PREDICTOR_CUBIC = 3,
-  FEM_PREDICTOR_QUARTIC = 4
+  FEM_PREDICTOR_QUARTIC = 4,
+  FEM_PREDICTOR_QUINTIC = 5,  /* streaming engine; the persistent engine uses quartic at most */
+  FEM_PREDICTOR_SEXTIC = 6    /* streaming engine; the persistent engine uses quartic at most */
 };
 
 /* How the workgroups of one team exchange vectors between barriers:

@@ -597,7 +597,7 @@ def test_fused_steps_with_and_without_sweeps(gpu, precision):
 
 
 @pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_LINEAR, _lib.FEM_PREDICTOR_QUADRATIC, _lib.FEM_PREDICTOR_CUBIC,
-                                  _lib.FEM_PREDICTOR_QUARTIC])
+                                  _lib.FEM_PREDICTOR_QUARTIC, _lib.FEM_PREDICTOR_QUINTIC, _lib.FEM_PREDICTOR_SEXTIC])
 @pytest.mark.parametrize("case", ["heat_persistent", "grayscott_persistent", "advdiff16_stream", "heat_stream",
                                   "newmark_persistent", "newmark_stream"])
 def test_predictors(gpu, case, pred):
@@ -676,14 +676,17 @@ def test_predictors(gpu, case, pred):
         assert rel(sol.get_state_unknowns(), u) < RTOL, n
 
 
-def test_history_across_engines_and_autotune(gpu):
+@pytest.mark.parametrize("pred", [_lib.FEM_PREDICTOR_CUBIC, _lib.FEM_PREDICTOR_SEXTIC])
+def test_history_across_engines_and_autotune(gpu, pred):
     """The streaming engine turns the predictor history by pointer, the
-    persistent kernel shifts it by copies: a cubic-predictor trajectory that
-    starts on the persistent kernel, goes through the autotuner's probes of
-    both engines (each restoring the state) and continues on the chosen one
-    stays on the oracle's trajectory, and so do stream launches of three, one
-    and two steps (turn counts that the three history slots do not divide)."""
-    print("case history across engines", flush=True)
+    persistent kernel shifts it by copies: a trajectory that starts on the
+    persistent kernel, goes through the autotuner's probes of both engines
+    (each restoring the state) and continues on the chosen one stays on the
+    oracle's trajectory, and so do stream launches of three, one and two steps
+    (turn counts that the history slots do not divide).  Sextic: the
+    persistent kernel uses quartic at most and leaves the two oldest slots to
+    the streaming engine."""
+    print("case history across engines", pred, flush=True)
     k, vel, prm = 16, (0.5, 0.3, 0.0), [1e-3, 3e-2]
     rng = np.random.default_rng(21)
     ctxs, systems, states = [], [], []
@@ -704,7 +707,7 @@ def test_history_across_engines_and_autotune(gpu):
             assert rel(b.get_state_unknowns(i).reshape(s.n, k), states[i]) < RTOL, (tag, i)
 
     def batch(engine):
-        b = Batch(ctxs, SolverOptions(nrhs=k, engine=engine, predictor=_lib.FEM_PREDICTOR_CUBIC))
+        b = Batch(ctxs, SolverOptions(nrhs=k, engine=engine, predictor=pred))
         for i in range(2):
             b.set_equation(i, AdvectionDiffusionParameters(*prm))
         return b
