@@ -537,9 +537,13 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0, gra
     ach = alg / (st_ms * 1e-3) / 1e9
     graph = None
     if engine == "stream" and graph_replay:
-        # the next K steps as replays of a recorded advance (sized from the
-        # eager runs), continuing the trajectory; a recording holds at most
-        # the library's launch limit, so G <= K steps are replayed K // G times
+        # the next K steps as replays of a recorded advance, continuing the
+        # trajectory; a recording is sized from the last eager advance, so a
+        # few eager steps first size it for the steps that follow (the long
+        # advance above includes the costly early steps); a recording holds at
+        # most the library's launch limit, so G <= K steps are replayed K // G
+        # times
+        b.advance(4)
         gsteps = steps
         while True:
             try:
@@ -563,10 +567,10 @@ def batched_measure(args, steps, warmup, plain_seconds=8.0, cpu_seconds=8.0, gra
         graph = {"value": dof * gsteps * reps / gw, "unit": "DOF·steps/s", "ms_per_step": 1e3 * gw / (gsteps * reps),
                  "status": gstatus, "iterations": gits, "steps_per_graph": gsteps, "replays": reps,
                  "algorithmic_GBs": galg / (gms * 1e-3) / 1e9,
-                 "note": "the next steps as replays of a HIP graph of every phase launch, with the sweeps and "
-                         "iterations of the eager steps plus a margin and no host reads (later steps need fewer "
-                         "iterations: compare algorithmic_GBs with roofline.achieved); wall time includes the "
-                         "host's per-replay statistics reads"}
+                 "note": "after 4 more eager steps, the next steps as replays of a HIP graph of every phase "
+                         "launch, with the sweeps and iterations of those 4 steps plus a margin and no host reads "
+                         "(later steps need fewer iterations: compare algorithmic_GBs with roofline.achieved); wall "
+                         "time includes the host's per-replay statistics reads"}
         log("batched: graph replays", reps, "x", gsteps, "steps", round(gw, 3), "s")
     out = {"value": dof * steps / wall, "unit": "DOF·steps/s", "steps": steps, "warmup": warmup,
            "ms_per_step": 1e3 * wall / steps, "status": int(status),

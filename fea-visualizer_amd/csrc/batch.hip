@@ -1376,9 +1376,12 @@ int stream_advance(fem_batch* b, int nsteps, fem_step_stats* stats) {
   rc = stream_run(b->splan, b->d_sinsts, max_sweeps, max_inner, nsteps, b->stream, &turned, &ctl);
   stream_turn_host(b, turned);
   if (rc) return rc;
-  // sweeps and iterations this plan needed, for a graph's budget
-  b->sobs_sweeps = std::max(b->sobs_sweeps, ctl.obs_sweeps);
-  for (int q = 0; q < kBudgetSweeps; q++) b->sobs_iters[q] = std::max(b->sobs_iters[q], ctl.obs_iters[q]);
+  // the sweeps and iterations this advance needed: a graph's budget (the
+  // most recent steps, so an advance just before a capture sizes it)
+  if (nsteps > 0) {
+    b->sobs_sweeps = ctl.obs_sweeps;
+    for (int q = 0; q < kBudgetSweeps; q++) b->sobs_iters[q] = ctl.obs_iters[q];
+  }
   FEM_HIP(hipEventRecord(b->ev1, b->stream));
   rc = read_info(b);
   if (rc) return rc;

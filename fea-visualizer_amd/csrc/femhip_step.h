@@ -168,8 +168,8 @@ struct fem_batch {
   long long graph_sgen = -1;
   femhip::Inst* graph_sinsts = nullptr;
   int graph_budget_sweeps = 0;
-  // most sweeps a step and most iterations each sweep took in eager
-  // streaming runs of the current plan (a streaming graph's budget)
+  // most sweeps a step and most iterations each sweep took in the last eager
+  // streaming advance of the current plan (a streaming graph's budget)
   int sobs_sweeps = 0;
   int sobs_iters[femhip::kBudgetSweeps] = {};
 };

@@ -332,10 +332,11 @@ int fem_batch_instance_stats(fem_batch* b, int32_t instance, fem_step_stats* sta
  * must be captured again.
  * Streaming engine: every phase, reduction and compaction launch of nsteps
  * steps, with nothing read on the host.  Each step issues the sweeps the
- * batch's eager advances needed plus one, each of the iterations that sweep
- * needed plus a quarter (at least 2); launches past an instance's end exit
- * on the device.  Capture therefore needs one eager fem_batch_advance first
- * (else FEM_ERR_STATE).  An instance that would need more iterations than
+ * batch's last eager advance needed (its costliest step) plus one, each of
+ * the iterations that sweep needed plus a quarter (at least 2); launches past
+ * an instance's end exit on the device.  Capture therefore needs one eager
+ * fem_batch_advance first (else FEM_ERR_STATE); a few eager steps just before
+ * the capture size it for the steps that follow.  An instance that would need more iterations than
  * recorded stops there and the replay reports FEM_NOT_CONVERGED; new
  * equation parameters are read by the next replay; a rebuilt row-block plan
  * (autotuning) makes the graph stale.  A recording holds at most 8192
