@@ -61,15 +61,20 @@ def run_driver(tmp_path, mesh, bc, values, equation, steps, *params):
     return read_output(fout, mesh["xyz"].shape[0])
 
 
-def oracle_frames(s, equation, prm, steps, field0, state, switch=None):
+def oracle_frames(s, equation, prm, steps, field0, state, switch=None, repaint=False):
     """The reference loop with float32 surface values between frames;
     switch = (frame, system): from that frame on, step with the other
-    system (re-assembled matrices)."""
+    system (re-assembled matrices); repaint: at that frame the solver's state
+    is cleared and the surface painted again with field0 (a boundary-condition
+    change: clear_solver, then a new unknown numbering)."""
     u, v = state
     surf = field0.astype(np.float32)
     for f in range(steps):
         if switch is not None and f == switch[0]:
             s = switch[1]
+            if repaint:
+                u, v = np.zeros(s.n), np.zeros(s.n)
+                surf = field0.astype(np.float32)
         x = s.gather(surf.astype(np.float64))
         if equation == oracle.GRAYSCOTT:
             v = x
@@ -97,6 +102,37 @@ def test_driver_frame_loop_matches_oracle(gpu, tmp_path, eq):
     onb = m["on_boundary"].astype(bool)
     assert np.all(got[onb] == 0.0)
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("eq", ["heat", "wave"])
+@pytest.mark.parametrize("bcs", [(0, 1), (1, 0)])
+def test_driver_boundary_condition_switch(gpu, tmp_path, eq, bcs):
+    """A boundary-condition change after 5 of 10 frames (Application.cpp:313-315:
+    clear_solver, then update_boundary_conditions, which re-numbers the
+    unknowns): the adapter rebuilds its context and solver, the surface is
+    painted again, and the frames continue with the other unknown count,
+    against the oracle doing the same."""
+    m = meshes.jittered_square(20, seed=9)
+    x = m["xyz"].astype(np.float64)
+    vals = np.exp(-((x[:, 0] - 0.3) ** 2 + (x[:, 1] - 0.45) ** 2) / 0.03)
+    prm = [1e-3, 1.0] if eq == "heat" else [5e-3, 0.8]
+    code = oracle.HEAT if eq == "heat" else oracle.WAVE
+    print("case driver bc switch", eq, bcs, flush=True)
+    frames, unstable, got = run_driver(tmp_path, m, bcs[0], vals, eq, 10, *prm, "--bc-at", 5, bcs[1])
+    assert frames == 10 and unstable == 0
+    s1, s2 = oracle.System(m, bc=bcs[0]), oracle.System(m, bc=bcs[1])
+    assert s1.n != s2.n
+    ref = oracle_frames(s1, code, prm, 10, vals, (np.zeros(s1.n), np.zeros(s1.n)), switch=(5, s2), repaint=True)
+    onb = m["on_boundary"].astype(bool)
+    if bcs[1] == 0:
+        assert np.all(got[onb] == 0.0)
+    else:
+        assert np.any(got[onb] != 0.0)
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    # staying on the first condition gives another field
+    stay = oracle_frames(s1, code, prm, 10, vals, (np.zeros(s1.n), np.zeros(s1.n)), switch=(5, s1), repaint=True)
+    assert np.linalg.norm(got - stay) > 1e-4 * np.linalg.norm(ref)
 
 
 @pytest.mark.gpu
