@@ -24,6 +24,7 @@
 #include "stream_vec.h"
 
 #include <algorithm>
+#include <chrono>
 
 namespace femhip {
 
@@ -741,6 +742,17 @@ const void* pick_reduce(int k, int ph) {
   }
 }
 
+// Publish one count read to pinned host memory: the values (a flag, or the
+// lists' counts), then the read's sequence number with a system-scope release
+// store, which the host polls.  A device-side write needs no copy command and
+// no event, whose completion signals cost more than the read itself.
+__global__ void k_post(const int* src, int nvals, unsigned* dst, unsigned seq) {
+  if (threadIdx.x != 0) return;
+  dst[1] = (unsigned)src[0];
+  dst[2] = nvals > 1 ? (unsigned)src[1] : 0u;
+  __hip_atomic_store(dst, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -750,8 +762,9 @@ const void* pick_reduce(int k, int ph) {
 bool stream_vec_layout(int k) { return k == 1 || k >= 4; }
 
 // Pinned slots for the host's reads of the list counts: a read is queued in
-// one slot while the previous one is still being waited for.
-constexpr int kCountSlots = 2;
+// one slot while the previous one is still being waited for.  Each slot holds
+// {sequence, instances, blocks, unused}.
+constexpr int kCountSlots = 2, kPostWords = 4;
 
 struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
@@ -767,8 +780,10 @@ struct StreamPlan {
   int* d_act_base = nullptr;  // [ninst]
   int* d_act_blk = nullptr;   // [nblocks]
   int* d_cnt = nullptr;       // {active instances, active blocks}
-  int* h_cnt = nullptr;       // pinned copies: kCountSlots x {instances, blocks}
-  hipEvent_t ev[kCountSlots] = {};  // marks each slot's copy
+  unsigned* h_post = nullptr;  // coherent pinned memory: kCountSlots x kPostWords, written by k_post
+  unsigned* d_post = nullptr;  // its device address
+  mutable unsigned post_seq = 0;                // sequence number of the last queued read
+  mutable unsigned slot_seq[kCountSlots] = {};  // the read each slot waits for
 };
 
 void stream_plan_free(StreamPlan* p) {
@@ -777,9 +792,7 @@ void stream_plan_free(StreamPlan* p) {
                   p->d_partials, p->d_act_inst, p->d_act_base, p->d_act_blk,  p->d_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (p->h_cnt) (void)hipHostFree(p->h_cnt);
-  for (hipEvent_t e : p->ev)
-    if (e) (void)hipEventDestroy(e);
+  if (p->h_post) (void)hipHostFree(p->h_post);
   delete p;
 }
 
@@ -841,11 +854,13 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_cnt = dalloc<int>(2, &rc);
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&p->h_cnt, 2 * kCountSlots * sizeof(int), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&p->h_post, kPostWords * kCountSlots * sizeof(unsigned),
+                                 hipHostMallocCoherent | hipHostMallocMapped);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
-    for (int q = 0; q < kCountSlots && !rc; q++) {
-      e = hipEventCreateWithFlags(&p->ev[q], hipEventDisableTiming);
-      if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate");
+    if (!rc) {
+      for (int q = 0; q < kPostWords * kCountSlots; q++) p->h_post[q] = 0;
+      e = hipHostGetDevicePointer((void**)&p->d_post, p->h_post, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipHostGetDevicePointer");
     }
   }
   if (rc) {
@@ -911,17 +926,27 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   return 0;
 }
 
+// Queue a read of nvals ints at src into pinned slot `slot` (k_post), under a
+// new sequence number that wait_counts polls for.
+int post_read(const StreamPlan* p, const int* src, int nvals, int slot, hipStream_t st) {
+  if (++p->post_seq == 0) ++p->post_seq;  // 0 is the slots' initial value
+  p->slot_seq[slot] = p->post_seq;
+  hipLaunchKernelGGL(k_post, dim3(1), dim3(kWave), 0, st, src, nvals, p->d_post + kPostWords * slot, p->post_seq);
+  FEM_HIP(hipGetLastError());
+  return 0;
+}
+
 // Compact the instances and row blocks that satisfy `pred` into the device
-// lists (k_compact, k_expand).  post >= 0: also queue the copy of the lists'
-// counts into that pinned slot and mark it with the slot's event (read back
-// by wait_counts).  The launch sizes in *cv are left as they are: a superset.
+// lists (k_compact, k_expand).  post >= 0: also queue the read of the lists'
+// counts into that pinned slot (read back by wait_counts).  The launch sizes
+// in *cv are left as they are: a superset.
 int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, int post = -1) {
   hipLaunchKernelGGL(k_compact, dim3(1), dim3(kBlock), 0, st, p->d_ss, p->d_inst_blk, p->ninst, pred, p->d_act_inst,
                      p->d_act_base, p->d_cnt);
   FEM_HIP(hipGetLastError());
   if (post >= 0) {
-    FEM_HIP(hipMemcpyAsync(p->h_cnt + 2 * post, p->d_cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    FEM_HIP(hipEventRecord(p->ev[post], st));
+    const int rc = post_read(p, p->d_cnt, 2, post, st);
+    if (rc) return rc;
   }
   if (p->nblocks > 0 && p->ninst > 1) {  // queued before the host waits, so it overlaps the read-back
     hipLaunchKernelGGL(k_expand, dim3(div_up(p->nblocks, kBlock)), dim3(kBlock), 0, st, p->d_blk_inst, p->d_inst_blk,
@@ -932,28 +957,48 @@ int compact(const StreamPlan* p, int pred, hipStream_t st, Cover* cv, int post =
   return 0;
 }
 
-// One instance: queue the copy of its sweep_done flag into `slot` instead of
+// One instance: queue the read of its sweep_done flag into `slot` instead of
 // a compaction (the kernels skip its blocks by the flag itself).
 int post_flag(const StreamPlan* p, int slot, hipStream_t st) {
-  FEM_HIP(hipMemcpyAsync(p->h_cnt + 2 * slot, (const char*)p->d_ss + offsetof(SState, sweep_done), sizeof(int),
-                         hipMemcpyDeviceToHost, st));
-  FEM_HIP(hipEventRecord(p->ev[slot], st));
-  return 0;
+  return post_read(p, (const int*)((const char*)p->d_ss + offsetof(SState, sweep_done)), 1, slot, st);
 }
 
 // Wait for the counts queued in `slot` (flag: a post_flag read, as counts):
-// nab == 0 means no block is left.
-int wait_counts(const StreamPlan* p, int slot, Cover* cv, bool flag = false) {
-  FEM_HIP(hipEventSynchronize(p->ev[slot]));
+// nab == 0 means no block is left.  The host polls the slot's sequence number;
+// past 1 ms it also asks whether the stream has drained, so that a read that
+// can no longer arrive (an earlier launch failed) ends the wait with an error.
+int wait_counts(const StreamPlan* p, int slot, hipStream_t st, Cover* cv, bool flag = false) {
+  const unsigned want = p->slot_seq[slot];
+  const unsigned* h = p->h_post + kPostWords * slot;
+  auto arrived = [&]() { return __atomic_load_n(h, __ATOMIC_ACQUIRE) == want; };
+  if (!arrived()) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto next_query = t0 + std::chrono::milliseconds(1);
+    for (unsigned spin = 1; !arrived(); spin++) {
+      if (spin % 256) continue;
+      const auto now = clk::now();
+      if (now < next_query) continue;
+      next_query = now + std::chrono::milliseconds(1);
+      const hipError_t e = hipStreamQuery(st);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return hip_fail(e, "hipStreamQuery");
+      if (arrived()) break;
+      set_error("stream engine: a count read did not arrive although the stream drained");
+      return FEM_ERR_HIP;
+    }
+  }
+  const int v0 = (int)__atomic_load_n(h + 1, __ATOMIC_RELAXED);
+  const int v1 = (int)__atomic_load_n(h + 2, __ATOMIC_RELAXED);
   cv->all = false;
   if (flag) {
-    const bool done = p->h_cnt[2 * slot] != 0;
+    const bool done = v0 != 0;
     cv->nact = done ? 0 : 1;
     cv->nab = done ? 0 : p->nblocks;
     return 0;
   }
-  cv->nact = p->h_cnt[2 * slot];
-  cv->nab = p->h_cnt[2 * slot + 1];
+  cv->nact = v0;
+  cv->nab = v1;
   return 0;
 }
 
@@ -1046,7 +1091,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       auto settle = [&]() -> int {
         if (sweep_slot < 0) return 0;
         Cover got{};
-        const int r = wait_counts(p, sweep_slot, &got, single);
+        const int r = wait_counts(p, sweep_slot, st, &got, single);
         sweep_slot = -1;
         none = got.nab == 0;
         if (!single) sweep = cv = got;
@@ -1081,7 +1126,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         }
         if (pend >= 0) {
           Cover got{};
-          if ((r = wait_counts(p, pend, &got, !multi))) return r;
+          if ((r = wait_counts(p, pend, st, &got, !multi))) return r;
           pend = -1;
           if (got.nab == 0) {  // done after iteration pend_it: this one is empty
             *stop = true;
