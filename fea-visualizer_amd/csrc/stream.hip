@@ -742,15 +742,17 @@ const void* pick_reduce(int k, int ph) {
   }
 }
 
-// Publish one count read to pinned host memory: the values (a flag, or the
-// lists' counts), then the read's sequence number with a system-scope release
-// store, which the host polls.  A device-side write needs no copy command and
-// no event, whose completion signals cost more than the read itself.
-__global__ void k_post(const int* src, int nvals, unsigned* dst, unsigned seq) {
+// Publish one count read to pinned host memory: each value (a flag, or the
+// lists' two counts) in a 64-bit word of its own, tagged with the read's
+// sequence number in the high half, which the host polls.  A word is written
+// by one aligned 8-byte store, so a reader sees a value and its tag together
+// and no release fence is needed (one at system scope writes back the L2).
+// A device-side write needs no copy command and no event.
+__global__ void k_post(const int* src, int nvals, unsigned long long* dst, unsigned seq) {
   if (threadIdx.x != 0) return;
-  dst[1] = (unsigned)src[0];
-  dst[2] = nvals > 1 ? (unsigned)src[1] : 0u;
-  __hip_atomic_store(dst, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  const unsigned long long tag = (unsigned long long)seq << 32;
+  for (int q = 0; q < nvals; q++)
+    __hip_atomic_store(dst + q, tag | (unsigned)src[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace
@@ -763,8 +765,8 @@ bool stream_vec_layout(int k) { return k == 1 || k >= 4; }
 
 // Pinned slots for the host's reads of the list counts: a read is queued in
 // one slot while the previous one is still being waited for.  Each slot holds
-// {sequence, instances, blocks, unused}.
-constexpr int kCountSlots = 2, kPostWords = 4;
+// two tagged words (instances, blocks; a flag uses the first).
+constexpr int kCountSlots = 2, kPostWords = 2;
 
 struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
@@ -780,10 +782,11 @@ struct StreamPlan {
   int* d_act_base = nullptr;  // [ninst]
   int* d_act_blk = nullptr;   // [nblocks]
   int* d_cnt = nullptr;       // {active instances, active blocks}
-  unsigned* h_post = nullptr;  // coherent pinned memory: kCountSlots x kPostWords, written by k_post
-  unsigned* d_post = nullptr;  // its device address
+  unsigned long long* h_post = nullptr;  // coherent pinned memory: kCountSlots x kPostWords, written by k_post
+  unsigned long long* d_post = nullptr;  // its device address
   mutable unsigned post_seq = 0;                // sequence number of the last queued read
   mutable unsigned slot_seq[kCountSlots] = {};  // the read each slot waits for
+  mutable int slot_vals[kCountSlots] = {};      // and how many words it writes
 };
 
 void stream_plan_free(StreamPlan* p) {
@@ -854,7 +857,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->d_act_blk = dalloc<int>(std::max(1, p->nblocks), &rc);
   p->d_cnt = dalloc<int>(2, &rc);
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&p->h_post, kPostWords * kCountSlots * sizeof(unsigned),
+    hipError_t e = hipHostMalloc((void**)&p->h_post, kPostWords * kCountSlots * sizeof(unsigned long long),
                                  hipHostMallocCoherent | hipHostMallocMapped);
     if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
     if (!rc) {
@@ -929,8 +932,9 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
 // Queue a read of nvals ints at src into pinned slot `slot` (k_post), under a
 // new sequence number that wait_counts polls for.
 int post_read(const StreamPlan* p, const int* src, int nvals, int slot, hipStream_t st) {
-  if (++p->post_seq == 0) ++p->post_seq;  // 0 is the slots' initial value
+  if (++p->post_seq == 0) ++p->post_seq;  // 0 is the slots' initial tag
   p->slot_seq[slot] = p->post_seq;
+  p->slot_vals[slot] = nvals;
   hipLaunchKernelGGL(k_post, dim3(1), dim3(kWave), 0, st, src, nvals, p->d_post + kPostWords * slot, p->post_seq);
   FEM_HIP(hipGetLastError());
   return 0;
@@ -964,13 +968,21 @@ int post_flag(const StreamPlan* p, int slot, hipStream_t st) {
 }
 
 // Wait for the counts queued in `slot` (flag: a post_flag read, as counts):
-// nab == 0 means no block is left.  The host polls the slot's sequence number;
-// past 1 ms it also asks whether the stream has drained, so that a read that
-// can no longer arrive (an earlier launch failed) ends the wait with an error.
+// nab == 0 means no block is left.  The host polls the words' tags; past 1 ms
+// it also asks whether the stream has drained, so that a read that can no
+// longer arrive (an earlier launch failed) ends the wait with an error.
 int wait_counts(const StreamPlan* p, int slot, hipStream_t st, Cover* cv, bool flag = false) {
   const unsigned want = p->slot_seq[slot];
-  const unsigned* h = p->h_post + kPostWords * slot;
-  auto arrived = [&]() { return __atomic_load_n(h, __ATOMIC_ACQUIRE) == want; };
+  const int nw = p->slot_vals[slot];
+  const unsigned long long* h = p->h_post + kPostWords * slot;
+  unsigned long long w[kPostWords] = {};
+  auto arrived = [&]() {
+    for (int q = 0; q < nw; q++) {
+      w[q] = __atomic_load_n(h + q, __ATOMIC_ACQUIRE);
+      if ((unsigned)(w[q] >> 32) != want) return false;
+    }
+    return true;
+  };
   if (!arrived()) {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
@@ -988,8 +1000,8 @@ int wait_counts(const StreamPlan* p, int slot, hipStream_t st, Cover* cv, bool f
       return FEM_ERR_HIP;
     }
   }
-  const int v0 = (int)__atomic_load_n(h + 1, __ATOMIC_RELAXED);
-  const int v1 = (int)__atomic_load_n(h + 2, __ATOMIC_RELAXED);
+  const int v0 = (int)(unsigned)w[0];
+  const int v1 = nw > 1 ? (int)(unsigned)w[1] : 0;
   cv->all = false;
   if (flag) {
     const bool done = v0 != 0;
