@@ -77,6 +77,8 @@ struct SArgs {
   int nall;             // row blocks of the plan (launches that cover all of them)
   int stage_rows;       // LDS stage capacity: rows and nonzeros per row block
   int stage_nnz;
+  unsigned long long* post;  // one instance: pinned word for the host's read of sweep_done, written by the
+  unsigned post_seq;         //   reduction that ends the iteration (null: no read), tagged with post_seq
 };
 
 // Nonzeros a row block may hold: blocks are cut so that their pattern fits
@@ -394,6 +396,19 @@ __device__ void sum_blocks(const SArgs& a, const int* inst_blk, int i, double* r
   __syncthreads();
 }
 
+// Phases whose reduction ends a Krylov iteration (and sets sweep_done).
+template <int PH>
+constexpr bool ends_iteration() {
+  return PH == PH_PCG_ITER || PH == PH_CG_B || PH == PH_BI_X;
+}
+
+// The host's read of one instance's sweep_done: one 8-byte word with the
+// read's sequence number in the high half (the layout k_post writes).
+__device__ inline void post_flag_word(const SArgs& a, int v) {
+  __hip_atomic_store(a.post, ((unsigned long long)a.post_seq << 32) | (unsigned)v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // One block per instance: reduce the phase's partials and advance the scalar
 // recurrences exactly as the persistent drivers do after their barriers.
 template <int K, int PH>
@@ -407,7 +422,11 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
   if constexpr (PH == PH_RES) {
     if (a.pass ? !S.redo : S.solve_done) return;
   } else if constexpr (PH != PH_RHS) {
-    if (S.sweep_done) return;
+    if (S.sweep_done) {
+      if constexpr (ends_iteration<PH>())
+        if (a.post && threadIdx.x == 0) post_flag_word(a, 1);
+      return;
+    }
   }
   sum_blocks<K, nsums<K, PH>()>(a, inst_blk, i, red, scratch);
   if (threadIdx.x != 0) return;
@@ -551,6 +570,8 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
     }
     S.sweep_done = alldone || brk || S.it_s >= I.max_inner || S.its >= I.max_iters;
   }
+  if constexpr (ends_iteration<PH>())
+    if (a.post) post_flag_word(a, S.sweep_done);
   (void)fp32;
 }
 
@@ -929,13 +950,19 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   return 0;
 }
 
-// Queue a read of nvals ints at src into pinned slot `slot` (k_post), under a
-// new sequence number that wait_counts polls for.
-int post_read(const StreamPlan* p, const int* src, int nvals, int slot, hipStream_t st) {
+// Take pinned slot `slot` for a read of nvals words under a new sequence
+// number, which wait_counts polls for.
+unsigned claim_slot(const StreamPlan* p, int slot, int nvals) {
   if (++p->post_seq == 0) ++p->post_seq;  // 0 is the slots' initial tag
   p->slot_seq[slot] = p->post_seq;
   p->slot_vals[slot] = nvals;
-  hipLaunchKernelGGL(k_post, dim3(1), dim3(kWave), 0, st, src, nvals, p->d_post + kPostWords * slot, p->post_seq);
+  return p->post_seq;
+}
+
+// Queue a read of nvals ints at src into pinned slot `slot` (k_post).
+int post_read(const StreamPlan* p, const int* src, int nvals, int slot, hipStream_t st) {
+  const unsigned seq = claim_slot(p, slot, nvals);
+  hipLaunchKernelGGL(k_post, dim3(1), dim3(kWave), 0, st, src, nvals, p->d_post + kPostWords * slot, seq);
   FEM_HIP(hipGetLastError());
   return 0;
 }
@@ -1114,26 +1141,44 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
       // kCheckEvery-th time queue another read (budgeted: refresh only)
       int since = 0, it_run = 0, pend = -1, pend_it = 0, need = 0;
       const int expect = sw < (int)prev_its.size() ? prev_its[sw] : 0;
+      // before an iteration's launches: whether it ends with a read (the slot,
+      // or -1); one instance with row blocks: the iteration's last reduction
+      // writes the flag (folded), otherwise k_post after the launches
+      int post = -1;
+      bool folded = false;
+      auto plan_read = [&]() {
+        post = -1;
+        folded = false;
+        a.post = nullptr;
+        if (budget) return;
+        // one instance with the previous step's count: checks before it rarely
+        // end the sweep, so they are sparser
+        const int every = single && expect > 0 ? 2 * kCheckEvery : kCheckEvery;
+        if (++since >= every || (expect > 0 && it_run + 1 >= expect - 1)) {
+          since = 0;
+          slot ^= 1;
+          post = slot;
+          if (single && p->nblocks > 0) {
+            a.post_seq = claim_slot(p, slot, 1);
+            a.post = p->d_post + kPostWords * slot;
+            folded = true;
+          }
+        }
+      };
       auto next = [&](bool* stop) -> int {
         ++it_run;
         *stop = false;
+        a.post = nullptr;
         if (budget) {
           if (single) return 0;
           const int r = compact(p, 0, st, &cv);
           cv = fixed;
           return r;
         }
-        const bool check = ++since >= kCheckEvery || (expect > 0 && it_run >= expect - 1);
-        int post = -1;
-        if (check) {
-          since = 0;
-          slot ^= 1;
-          post = slot;
-        }
         int r = 0;
         if (multi) {
           if ((r = compact(p, 0, st, &cv, post))) return r;
-        } else if (check && (r = post_flag(p, post, st))) {
+        } else if (post >= 0 && !folded && (r = post_flag(p, post, st))) {
           return r;
         }
         if (pend >= 0) {
@@ -1168,6 +1213,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         for (int it = 0; it < nits && !stop && !none; it++) {
           a.par = par;
           a.first = it == 0;
+          plan_read();
           STREAM_PH(PH_PCG_ITER, cv);
           par ^= 1;
           if ((rc = next(&stop))) return rc;
@@ -1176,6 +1222,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
         for (int it = 0; it < nits && !stop && !none; it++) {
           a.par = par;
           a.first = it == 0;
+          plan_read();
           if (p->kr == 2) {
             STREAM_PH(PH_CG_A, cv);
             STREAM_PH(PH_CG_B, cv);
@@ -1187,7 +1234,10 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
           par ^= 1;
           if (it == 0) {
             if ((rc = settle())) return rc;
-            if (none) break;
+            if (none) {
+              a.post = nullptr;
+              break;
+            }
           }
           if ((rc = next(&stop))) return rc;
         }
