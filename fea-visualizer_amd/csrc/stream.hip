@@ -79,6 +79,7 @@ struct SArgs {
   int stage_nnz;
   unsigned long long* post;  // one instance: pinned word for the host's read of sweep_done, written by the
   unsigned post_seq;         //   reduction that ends the iteration (null: no read), tagged with post_seq
+  int rhs_stage;             // right-hand side: the launch carries the FP64 stage (steady cases staged)
 };
 
 // Nonzeros a row block may hold: blocks are cut so that their pattern fits
@@ -229,14 +230,17 @@ __device__ void stream_update_newmark(const Inst& I, int row_lo, int row_hi, con
 // gathering Krylov phases.
 template <int K, int V, typename TI, int PH>
 __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int row_lo, int row_hi,
-                           const Stage<TI>& g) {
+                           const Stage<TI>& g, int rcase = 0, const RhsStage& rs = RhsStage{}) {
   constexpr int NS = nsums<K, PH>();
   const bool first = a.first != 0;
   const int par = a.par;
   vec<double, V> acc[NS];
   for (int q = 0; q < NS; q++) acc[q] = splat<V>(0.0);
   if constexpr (PH == PH_RHS) {
-    vphase_rhs<K, V, TI>(I, row_lo, row_hi, a.step, acc);
+    if (rcase)
+      vphase_rhs_staged<K, V, TI>(I, row_lo, row_hi, rcase, rs, acc);
+    else
+      vphase_rhs<K, V, TI>(I, row_lo, row_hi, a.step, acc);
   } else if constexpr (PH == PH_RES) {
     vphase_res<K, V, TI>(I, row_lo, row_hi, sh, acc);
   } else if constexpr (PH == PH_PCG_INIT) {
@@ -343,6 +347,35 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
       }
       __syncthreads();
       run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
+    } else if constexpr (vec_layout<K>() && PH == PH_RHS) {
+      // steady cases: stage the block's pattern and its two FP64 value rows
+      // (K or M, and S), which one lane per row would read with many
+      // distinct lines per load
+      const int rcase = a.rhs_stage ? rhs_case(I, a.step) : 0;
+      if (rcase) {
+        extern __shared__ __attribute__((aligned(16))) int dyn[];
+        const int GA* rp = gp(I.rp);
+        const int s_lo = rp[row_lo], s_hi = rp[row_hi];
+        const int nrow = row_hi - row_lo, nnz = s_hi - s_lo;  // within the stage (plan)
+        int LA* lrp = (int LA*)dyn;
+        int LA* lci = lrp + ((a.stage_rows + 4) & ~3);
+        double LA* la = (double LA*)(lci + ((a.stage_nnz + 3) & ~3));
+        double LA* ls = la + a.stage_nnz;
+        const int GA* ci = gp(I.ci);
+        const double GA* wa = gp(rcase == 1 ? I.K64 : I.M64);
+        const double GA* ws = gp(I.S64);
+        for (int t = threadIdx.x; t <= nrow; t += kBlock) lrp[t] = rp[row_lo + t] - s_lo;
+        for (int t = threadIdx.x; t < nnz; t += kBlock) {
+          lci[t] = ci[s_lo + t];
+          la[t] = wa[s_lo + t];
+          ls[t] = ws[s_lo + t];
+        }
+        __syncthreads();
+        run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{}, rcase, RhsStage{lrp, lci, la, ls});
+      } else {
+        __syncthreads();
+        run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
+      }
     } else if constexpr (vec_layout<K>()) {
       __syncthreads();
       run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{});
@@ -793,6 +826,7 @@ struct StreamPlan {
   int ninst = 0, nblocks = 0, k = 1, fp32 = 1, kr = 0, rows = 0;
   int stage_rows = 0, stage_nnz = 0;  // largest row block: rows, nonzeros (up to kStageCap)
   size_t lds = 0;                     // dynamic LDS of the staged phases
+  size_t lds_rhs = 0;                 // and of the right-hand side's FP64 stage (0: not staged)
   SState* d_ss = nullptr;
   int* d_blk_inst = nullptr;
   int* d_blk_row = nullptr;
@@ -866,6 +900,10 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
     set_error("stream engine: a matrix row has too many nonzeros for the LDS stage; use the persistent engine");
     return FEM_ERR_INVALID;
   }
+  // the right-hand side's stage: pattern and two FP64 value rows, where it fits
+  const size_t lds_rhs =
+      4 * (size_t)((p->stage_rows + 4) & ~3) + 4 * (size_t)((p->stage_nnz + 3) & ~3) + 16 * (size_t)p->stage_nnz;
+  p->lds_rhs = stream_vec_layout(k) && lds_rhs <= kMaxLds ? lds_rhs : 0;
   int rc = 0;
   p->d_ss = dalloc<SState>(p->ninst, &rc);
   p->d_blk_inst = dalloc<int>(std::max(1, p->nblocks), &rc);
@@ -940,7 +978,8 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   void* args[] = {(void*)&aa};
   const bool staged = stream_vec_layout(p->k) && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A ||
                                                   ph == PH_BI_P || ph == PH_BI_S);
-  FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, staged ? p->lds : 0, st));
+  const size_t lds = staged ? p->lds : ph == PH_RHS ? p->lds_rhs : 0;
+  FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, lds, st));
   if (ph == PH_UPD) return 0;
   const void* rf = pick_reduce(p->k, ph);
   const int* ib = p->d_inst_blk;
@@ -1064,6 +1103,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   a.nall = p->nblocks;
   a.stage_rows = p->stage_rows;
   a.stage_nnz = p->stage_nnz;
+  a.rhs_stage = p->lds_rhs > 0;
   a.partials = p->d_partials;
   hipLaunchKernelGGL(k_hist_load, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, const_cast<Inst*>(d_insts),
                      p->ninst);

@@ -143,6 +143,67 @@ struct Stage {
   int row_lo;
 };
 
+// The right-hand side's steady cases after a launch's first step, where the
+// previous update formed what the rows gather (vphase_rhs's nm and pre): 1
+// Newmark (K and S rows, gathering hvec and the guess in x), 2 heat or
+// advection-diffusion with a predictor (M and S rows, gathering u and the
+// guess in x), 0 otherwise.
+__device__ inline int rhs_case(const Inst& I, int step) {
+  if (step <= 0) return 0;
+  if (I.eq == EQ_NEWMARK) return I.hvec ? 1 : 0;
+  return ((I.eq == EQ_HEAT || I.eq == EQ_ADVDIFF) && I.warm && I.extrap > 0) ? 2 : 0;
+}
+
+// A row block's pattern and its two FP64 value rows staged in LDS (the
+// right-hand side's steady cases): local row pointers, columns, the K or M
+// entries and the S entries.
+struct RhsStage {
+  const int LA* rp;
+  const int LA* ci;
+  const double LA* a;
+  const double LA* s;
+};
+
+// vphase_rhs for rhs_case(I, step) = cs != 0 from the staged rows: the same
+// per-row arithmetic, with the matrix rows read from LDS (one coalesced copy
+// per block instead of one row per lane from memory).
+template <int K, int V, typename TI>
+__device__ void vphase_rhs_staged(const Inst& I, int row_lo, int row_hi, int cs, const RhsStage& g,
+                                  vec<double, V> (&acc)[3]) {
+  using DV = vec<double, V>;
+  using TV = vec<TI, V>;
+  using L = VLay<K, V>;
+  constexpr int U = vunroll<V>();
+  const int j0 = ((int)threadIdx.x % L::G) * V;
+  TI* r = (TI*)I.r;
+  TI* xi = (TI*)I.xi;
+  const TI GA* dinv = gp((const TI*)I.dinvi);
+  const bool warm = I.warm != 0;
+  const bool pre = cs == 2 || warm;  // x holds the guess already: not stored
+  const double scale = cs == 1 ? -I.p1 * I.p1 : 1.0 / I.dt;
+  const double* gv = cs == 1 ? I.hvec : I.u;
+  const bool gs = cs == 2 || warm;  // S x is gathered
+  const DV zd = splat<V>(0.0);
+  for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
+    const size_t o = (size_t)row * K + j0;
+    const P2<DV> q = vgather<U>(g.rp[row - row_lo], g.rp[row - row_lo + 1], P2<DV>{zd, zd}, [&](int e) {
+      const size_t oc = (size_t)g.ci[e] * K + j0;
+      return P2<DV>{g.a[e] * vld<V>(gv + oc), gs ? g.s[e] * vld<V>(I.x + oc) : zd};
+    });
+    const DV b = cs == 1 ? scale * q.a : q.a * scale;
+    const DV r64 = b - q.b;
+    vst<V>(I.b + o, b);
+    if (!pre) vst<V>(I.x + o, zd);
+    const TV rr = to_i<TI, V>(r64);
+    vst<V>(r + o, rr);
+    vst<V>(xi + o, splat<V>((TI)0));
+    acc[0] += b * b;
+    acc[1] += r64 * r64;
+    const DV rd = to_d<V>(rr);
+    acc[2] += (I.krylov == 1) ? rd * rd : rd * to_d<V>(dinv[row] * rr);
+  }
+}
+
 template <int K, int V, typename TI>
 __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<double, V> (&acc)[3]) {
   using DV = vec<double, V>;
