@@ -79,7 +79,8 @@ struct SArgs {
   int stage_nnz;
   unsigned long long* post;  // one instance: pinned word for the host's read of sweep_done, written by the
   unsigned post_seq;         //   reduction that ends the iteration (null: no read), tagged with post_seq
-  int rhs_stage;             // right-hand side: the launch carries the FP64 stage (steady cases staged)
+  int rhs_stage;             // right-hand side: the launch carries the FP64 stage
+  int res_stage;             // FP64 residual: likewise
 };
 
 // Nonzeros a row block may hold: blocks are cut so that their pattern fits
@@ -242,7 +243,10 @@ __device__ void run_vector(const Inst& I, const SArgs& a, Shared& sh, int b, int
     else
       vphase_rhs<K, V, TI>(I, row_lo, row_hi, a.step, acc);
   } else if constexpr (PH == PH_RES) {
-    vphase_res<K, V, TI>(I, row_lo, row_hi, sh, acc);
+    if (rcase)
+      vphase_res<K, V, TI, true>(I, row_lo, row_hi, sh, acc, rs);
+    else
+      vphase_res<K, V, TI>(I, row_lo, row_hi, sh, acc);
   } else if constexpr (PH == PH_PCG_INIT) {
     vphase_pcg_init<K, V, TI>(I, row_lo, row_hi, g, (TI*)I.r, (TI*)I.q, acc);
   } else if constexpr (PH == PH_PCG_ITER) {
@@ -347,11 +351,11 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
       }
       __syncthreads();
       run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{lrp, lci, lw, row_lo});
-    } else if constexpr (vec_layout<K>() && PH == PH_RHS) {
-      // steady cases: stage the block's pattern and its two FP64 value rows
-      // (K or M, and S), which one lane per row would read with many
-      // distinct lines per load
-      const int rcase = a.rhs_stage ? rhs_case(I, a.step) : 0;
+    } else if constexpr (vec_layout<K>() && (PH == PH_RHS || PH == PH_RES)) {
+      // right-hand side (steady cases) and FP64 residual: stage the block's
+      // pattern and its FP64 value rows (K or M, and S; the residual: S),
+      // which one lane per row would read with many distinct lines per load
+      const int rcase = PH == PH_RES ? (a.res_stage ? 3 : 0) : a.rhs_stage ? rhs_case(I, a.step) : 0;
       if (rcase) {
         extern __shared__ __attribute__((aligned(16))) int dyn[];
         const int GA* rp = gp(I.rp);
@@ -362,13 +366,13 @@ __global__ void __launch_bounds__(kBlock) k_sphase(SArgs a) {
         double LA* la = (double LA*)(lci + ((a.stage_nnz + 3) & ~3));
         double LA* ls = la + a.stage_nnz;
         const int GA* ci = gp(I.ci);
-        const double GA* wa = gp(rcase == 1 ? I.K64 : I.M64);
+        const double GA* wa = gp(rcase == 1 ? I.K64 : rcase == 2 ? I.M64 : I.S64);
         const double GA* ws = gp(I.S64);
         for (int t = threadIdx.x; t <= nrow; t += kBlock) lrp[t] = rp[row_lo + t] - s_lo;
         for (int t = threadIdx.x; t < nnz; t += kBlock) {
           lci[t] = ci[s_lo + t];
           la[t] = wa[s_lo + t];
-          ls[t] = ws[s_lo + t];
+          if (rcase != 3) ls[t] = ws[s_lo + t];
         }
         __syncthreads();
         run_vector<K, V, TI, PH>(I, a, sh, b, row_lo, row_hi, Stage<TI>{}, rcase, RhsStage{lrp, lci, la, ls});
@@ -827,6 +831,7 @@ struct StreamPlan {
   int stage_rows = 0, stage_nnz = 0;  // largest row block: rows, nonzeros (up to kStageCap)
   size_t lds = 0;                     // dynamic LDS of the staged phases
   size_t lds_rhs = 0;                 // and of the right-hand side's FP64 stage (0: not staged)
+  size_t lds_res = 0;                 // and of the FP64 residual's (one value row)
   SState* d_ss = nullptr;
   int* d_blk_inst = nullptr;
   int* d_blk_row = nullptr;
@@ -870,6 +875,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   p->kr = kr;
   p->rows = rows;
   std::vector<int> bi, br, be, ib(p->ninst + 1, 0);
+  double s_bytes = 0.0;  // FP64 S rows of all instances
   for (int i = 0; i < p->ninst; i++) {
     ib[i] = (int)bi.size();
     const fem_ctx* c = insts[i].ctx;
@@ -890,6 +896,7 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
       p->stage_nnz = std::max(p->stage_nnz, rp[e] - rp[r]);
       r = e;
     }
+    s_bytes += 8.0 * rp[n];
   }
   ib[p->ninst] = (int)bi.size();
   p->nblocks = (int)bi.size();
@@ -904,6 +911,10 @@ int stream_plan_build(const std::vector<InstHost>& insts, const Inst* d_insts, i
   const size_t lds_rhs =
       4 * (size_t)((p->stage_rows + 4) & ~3) + 4 * (size_t)((p->stage_nnz + 3) & ~3) + 16 * (size_t)p->stage_nnz;
   p->lds_rhs = stream_vec_layout(k) && lds_rhs <= kMaxLds ? lds_rhs : 0;
+  // the residual's S rows: staged when they exceed half of the 256 MB
+  // Infinity Cache (they then come from HBM); below that they stay cached
+  // between sweeps and one lane per row reads them faster than a staged copy
+  p->lds_res = p->lds_rhs && s_bytes > 128.0 * (1 << 20) ? lds_rhs - 8 * (size_t)p->stage_nnz : 0;
   int rc = 0;
   p->d_ss = dalloc<SState>(p->ninst, &rc);
   p->d_blk_inst = dalloc<int>(std::max(1, p->nblocks), &rc);
@@ -978,7 +989,7 @@ int launch_phase(const StreamPlan* p, const SArgs& a, int ph, const Cover& cv, h
   void* args[] = {(void*)&aa};
   const bool staged = stream_vec_layout(p->k) && (ph == PH_PCG_INIT || ph == PH_PCG_ITER || ph == PH_CG_A ||
                                                   ph == PH_BI_P || ph == PH_BI_S);
-  const size_t lds = staged ? p->lds : ph == PH_RHS ? p->lds_rhs : 0;
+  const size_t lds = staged ? p->lds : ph == PH_RHS ? p->lds_rhs : ph == PH_RES ? p->lds_res : 0;
   FEM_HIP(hipLaunchKernel(fn, dim3(nab), dim3(kBlock), args, lds, st));
   if (ph == PH_UPD) return 0;
   const void* rf = pick_reduce(p->k, ph);
@@ -1104,6 +1115,7 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
   a.stage_rows = p->stage_rows;
   a.stage_nnz = p->stage_nnz;
   a.rhs_stage = p->lds_rhs > 0;
+  a.res_stage = p->lds_res > 0;
   a.partials = p->d_partials;
   hipLaunchKernelGGL(k_hist_load, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, const_cast<Inst*>(d_insts),
                      p->ninst);

@@ -340,9 +340,11 @@ __device__ void vphase_rhs(const Inst& I, int row_lo, int row_hi, int step, vec<
 }
 
 // FP64 refinement residual r = b - S64 (x + xi); columns flagged in sh.reset
-// restart from x alone (their correction is dropped).
-template <int K, int V, typename TI>
-__device__ void vphase_res(const Inst& I, int row_lo, int row_hi, const Shared& sh, vec<double, V> (&acc)[2]) {
+// restart from x alone (their correction is dropped).  ST: the block's
+// pattern and S rows come from the LDS stage g (columns g.ci, entries g.a).
+template <int K, int V, typename TI, bool ST = false>
+__device__ void vphase_res(const Inst& I, int row_lo, int row_hi, const Shared& sh, vec<double, V> (&acc)[2],
+                           const RhsStage& g = RhsStage{}) {
   using DV = vec<double, V>;
   using TV = vec<TI, V>;
   using L = VLay<K, V>;
@@ -358,23 +360,26 @@ __device__ void vphase_res(const Inst& I, int row_lo, int row_hi, const Shared& 
   const int GA* __restrict__ rp = gp(I.rp);
   const int GA* __restrict__ ci = gp(I.ci);
   const double GA* S64 = gp(I.S64);
+  auto col = [&](int s) { if constexpr (ST) return g.ci[s]; else return ci[s]; };
+  auto val = [&](int s) { if constexpr (ST) return g.a[s]; else return S64[s]; };
   const DV zd = splat<V>(0.0);
   for (int row = row_lo + (int)threadIdx.x / L::G; row < row_hi; row += L::RPP) {
     const size_t o = (size_t)row * K + j0;
     const DV bo = vld<V>(I.b + o);
+    const int lo = ST ? g.rp[row - row_lo] : rp[row], hi = ST ? g.rp[row - row_lo + 1] : rp[row + 1];
     DV sx;
     if (!drop) {
-      sx = vgather<U>(rp[row], rp[row + 1], zd, [&](int s) {
-        const size_t oc = (size_t)ci[s] * K + j0;
-        return S64[s] * (vld<V>(I.x + oc) + to_d<V>(vld<V>(xi + oc)));
+      sx = vgather<U>(lo, hi, zd, [&](int s) {
+        const size_t oc = (size_t)col(s) * K + j0;
+        return val(s) * (vld<V>(I.x + oc) + to_d<V>(vld<V>(xi + oc)));
       });
     } else {
-      sx = vgather<U>(rp[row], rp[row + 1], zd, [&](int s) {
-        const size_t oc = (size_t)ci[s] * K + j0;
+      sx = vgather<U>(lo, hi, zd, [&](int s) {
+        const size_t oc = (size_t)col(s) * K + j0;
         DV c = to_d<V>(vld<V>(xi + oc));
 #pragma unroll
         for (int v = 0; v < V; v++) c[v] = dropv[v] ? 0.0 : c[v];
-        return S64[s] * (vld<V>(I.x + oc) + c);
+        return val(s) * (vld<V>(I.x + oc) + c);
       });
     }
     const DV r64 = bo - sx;
