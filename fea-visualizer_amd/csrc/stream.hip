@@ -465,6 +465,25 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
       return;
     }
   }
+  // pipelined CG, few columns: thread 0 loads the recurrence's scalars before
+  // the sums, so the loads overlap them (no dependent round trip at the end)
+  constexpr bool pre = PH == PH_PCG_ITER && K <= 4;
+  double p_gamma[pre ? K : 1], p_alpha[pre ? K : 1], p_ithr[pre ? K : 1];
+  int p_active[pre ? K : 1], p_it_s = 0, p_its = 0, p_max_inner = 0, p_max_iters = 0;
+  if constexpr (pre) {
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < K; j++) {
+        p_gamma[j] = S.gamma[j];
+        p_alpha[j] = S.alpha[j];
+        p_ithr[j] = S.ithr[j];
+        p_active[j] = S.active[j];
+      }
+      p_it_s = S.it_s;
+      p_its = S.its;
+      p_max_inner = I.max_inner;
+      p_max_iters = I.max_iters;
+    }
+  }
   sum_blocks<K, nsums<K, PH>()>(a, inst_blk, i, red, scratch);
   if (threadIdx.x != 0) return;
   if constexpr (PH == PH_RHS) {
@@ -514,13 +533,14 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
       S.beta[j] = 0.0;
     }
   } else if constexpr (PH == PH_PCG_ITER) {
-    S.it_s++;
-    S.its++;
+    const int it_s = (pre ? p_it_s : S.it_s) + 1, its = (pre ? p_its : S.its) + 1;
+    S.it_s = it_s;
+    S.its = its;
     int alldone = 1;
     for (int j = 0; j < K; j++) {
       const double g = red[j], d = red[K + j], rr = red[2 * K + j];
-      const double gold = S.gamma[j], aold = S.alpha[j];
-      const bool done = !S.active[j] || rr < S.ithr[j] || !(rr == rr);
+      const double gold = pre ? p_gamma[j] : S.gamma[j], aold = pre ? p_alpha[j] : S.alpha[j];
+      const bool done = !(pre ? p_active[j] : S.active[j]) || rr < (pre ? p_ithr[j] : S.ithr[j]) || !(rr == rr);
       double be = 0.0, al = 0.0;
       if (!done && gold != 0.0 && aold != 0.0) {
         be = g / gold;
@@ -532,7 +552,7 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
       S.gamma[j] = g;
       alldone &= done;
     }
-    S.sweep_done = alldone || S.it_s >= I.max_inner || S.its >= I.max_iters;
+    S.sweep_done = alldone || it_s >= (pre ? p_max_inner : I.max_inner) || its >= (pre ? p_max_iters : I.max_iters);
   } else if constexpr (PH == PH_CG_A) {
     for (int j = 0; j < K; j++) {
       const double pq = red[j];
