@@ -634,19 +634,26 @@ __global__ void __launch_bounds__(kRedBlock) k_sreduce(SArgs a, const int* inst_
 
 // Start of a refinement sweep: inner threshold, iteration counter, and the
 // decision whether the instance sweeps at all.
-__global__ void k_sweep_begin(SState* ss, const Inst* insts, int ninst, int k, int fp32) {
+// post (one instance, else null): the host's read of the sweep_done flag,
+// written here as k_post would write it.
+__global__ void k_sweep_begin(SState* ss, const Inst* insts, int ninst, int k, int fp32, unsigned long long* post,
+                              unsigned seq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   SState& S = ss[i];
   const Inst& I = insts[i];
   S.it_s = 0;
+  int done = 1;
   if (S.solve_done || S.sweeps >= I.max_sweeps || S.its >= I.max_iters) {
-    S.sweep_done = 1;
     S.solve_done = 1;
-    return;
+  } else {
+    for (int j = 0; j < k; j++) S.ithr[j] = fp32 ? fmax(I.inner_tol2 * S.ithr[j], S.thr[j]) : S.thr[j];
+    done = 0;
   }
-  for (int j = 0; j < k; j++) S.ithr[j] = fp32 ? fmax(I.inner_tol2 * S.ithr[j], S.thr[j]) : S.thr[j];
-  S.sweep_done = 0;
+  S.sweep_done = done;
+  if (post && i == 0)
+    __hip_atomic_store(post, ((unsigned long long)seq << 32) | (unsigned)done, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // End of a step: counters, convergence status, last residual, and the turn
@@ -1173,21 +1180,25 @@ int stream_run(const StreamPlan* p, const Inst* d_insts, int max_sweeps, int max
     STREAM_PH(PH_RHS, full);
     Cover prev_sweep = every;  // a superset of the next sweep's instances
     for (int sw = 0; sw < nsweeps; sw++) {
+      // one instance, eager: k_sweep_begin writes the read of its flag
+      int sweep_slot = -1;
+      unsigned long long* sb_post = nullptr;
+      unsigned sb_seq = 0;
+      if (single && !budget) {
+        slot ^= 1;
+        sweep_slot = slot;
+        sb_seq = claim_slot(p, slot, 1);
+        sb_post = p->d_post + kPostWords * slot;
+      }
       hipLaunchKernelGGL(k_sweep_begin, dim3(div_up(p->ninst, kBlock)), dim3(kBlock), 0, st, p->d_ss, d_insts,
-                         p->ninst, p->k, p->fp32);
+                         p->ninst, p->k, p->fp32, sb_post, sb_seq);
       FEM_HIP(hipGetLastError());
       // instances that sweep; eager: their counts arrive after the sweep's
       // first launches, which run over the previous sweep's cover meanwhile
       // (one instance: every launch covers all of its blocks, which skip by
       // its flags, and the host reads its sweep_done flag)
-      int sweep_slot = -1;
       if (single) {
         cv = full;
-        if (!budget) {
-          slot ^= 1;
-          sweep_slot = slot;
-          if ((rc = post_flag(p, sweep_slot, st))) return rc;
-        }
       } else if (budget) {
         if ((rc = compact(p, 0, st, &cv))) return rc;
         cv = fixed;
